@@ -1,0 +1,171 @@
+/*
+ * psdp.h -- C ABI of libpsdp, the MI355X (gfx950) PSD-projection library.
+ *
+ * Every entry point is `extern "C"`, takes plain device pointers and sizes,
+ * and returns a psdp_status_t.  Matrices are dense, symmetric, column-major
+ * n x n with leading dimension n, resident in device memory.  Work is
+ * enqueued on the handle's HIP stream.
+ *
+ * Drop-in map (reference = ComputationalRobotics/psd_projection,
+ * include/psd_projection/*.h; cuBLAS/cuSOLVER handles become one
+ * psdp_handle_t; `float* workspace` becomes (ws, ws_bytes)):
+ *
+ *   psdp_composite_fp32                  <- composite_FP32              composite_FP32.h:12-17
+ *   psdp_composite_fp32_auto_scale       <- composite_FP32_auto_scale   composite_FP32.h:24-30
+ *   psdp_composite_fp16                  <- composite_FP16              composite_FP16.h:12-18
+ *   psdp_composite_fp16_auto_scale       <- composite_FP16_auto_scale   composite_FP16.h:26-33
+ *   psdp_composite_fp32_emulated         <- composite_FP32_emulated     composite_FP32_emulated.h:17-22
+ *   psdp_composite_fp32_emulated_auto_scale
+ *                                        <- composite_FP32_emulated_auto_scale
+ *                                                                       composite_FP32_emulated.h:29-35
+ *   psdp_approximate_two_norm            <- approximate_two_norm        lanczos.h:11-17
+ *   psdp_eig_fp64_psd                    <- eig_FP64_psd                eig_FP64_psd.h:14-20
+ *   psdp_eig_fp32_psd                    <- eig_FP32_psd                eig_FP32_psd.h:14-20
+ *
+ * Additions with no reference counterpart: float32 in/out projections with a
+ * per-step precision schedule (psdp_project_f32/_f64), grouped projection of
+ * many blocks (psdp_project_batched_f32), workspace sizing and the handle.
+ *
+ * Error convention: the reference prints CHECK_* failures and continues
+ * (include/psd_projection/check.h:18-57); here every call returns a status and
+ * psdp_last_error() describes the last failure of the calling thread.
+ */
+#ifndef PSD_PROJECTION_PSDP_H
+#define PSD_PROJECTION_PSDP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum psdp_status {
+  PSDP_OK = 0,
+  PSDP_ERR_INVALID_ARG = 1,
+  PSDP_ERR_NOMEM = 2,
+  PSDP_ERR_NONFINITE = 3,
+  PSDP_ERR_UNSUPPORTED = 4,
+  PSDP_ERR_DEVICE = 5
+} psdp_status_t;
+
+/* Operand format of the products of one filter step. */
+typedef enum psdp_format {
+  PSDP_FMT_F32 = 0,    /* float32 operands on the FP32 MFMA                       */
+  PSDP_FMT_F16 = 1,    /* binary16 operands, float32 accumulation                  */
+  PSDP_FMT_F16X3 = 2,  /* hi + 2^-11 lo binary16 split, 3 products (~22-bit)       */
+  PSDP_FMT_BF16 = 3,   /* bfloat16 operands                                        */
+  PSDP_FMT_BF16X3 = 4, /* hi + lo bfloat16 split, 3 products (~16-bit)             */
+  PSDP_FMT_BF16X9 = 5  /* hi + mid + lo bfloat16 split, 9 products (~fp32)         */
+} psdp_format_t;
+
+/* Coefficient table of the composite polynomial chain. */
+typedef enum psdp_table {
+  PSDP_TABLE_FP32 = 0, /* 10 steps, src/composite_FP32.cu:73-86 (scale 1.001, steps 0-7) */
+  PSDP_TABLE_FP16 = 1  /* 7 steps,  src/composite_FP16.cu:93-103 (scale 1.01, all steps) */
+} psdp_table_t;
+
+/* Named methods (the reference's MEX method strings plus the shipped
+ * mixed-precision schedule). */
+typedef enum psdp_method {
+  PSDP_METHOD_COMPOSITE_FP32 = 0,          /* FP32 table, FP32 MFMA everywhere          */
+  PSDP_METHOD_COMPOSITE_FP16 = 1,          /* FP16 table, FP16 operands everywhere      */
+  PSDP_METHOD_COMPOSITE_FP32_EMULATED = 2, /* FP32 table, BF16x9 everywhere             */
+  PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 steps 0-2, FP16 after  */
+  PSDP_METHOD_CUSTOM = 4                   /* table + formats taken from psdp_config_t  */
+} psdp_method_t;
+
+#define PSDP_MAX_STEPS 16
+
+typedef struct psdp_config {
+  int method;                     /* psdp_method_t                                   */
+  int table;                      /* psdp_table_t (CUSTOM only)                      */
+  int step_format[PSDP_MAX_STEPS];/* psdp_format_t per step (CUSTOM only)            */
+  int final_format;               /* psdp_format_t of W = X S (CUSTOM only)          */
+  int auto_scale;                 /* 1: Lanczos bound + rescale (the *_auto_scale)   */
+  int lanczos_iters;              /* default 20 (lanczos.h:16), at most 64           */
+  double lanczos_tol;             /* default 1e-10 (lanczos.h:16)                    */
+  double scale_margin;            /* s = up * margin; default 1.0 (reference)        */
+  int synchronize;                /* 1: wait for the stream and check device flags   */
+} psdp_config_t;
+
+typedef struct psdp_handle_st* psdp_handle_t;
+
+/* Handle: device, stream (default: a new non-blocking stream owned by the
+ * handle), workspace arena, Lanczos state, cached tile lists. */
+psdp_status_t psdp_create(psdp_handle_t* handle, int device);
+psdp_status_t psdp_destroy(psdp_handle_t handle);
+psdp_status_t psdp_set_stream(psdp_handle_t handle, void* hip_stream);
+psdp_status_t psdp_get_stream(psdp_handle_t handle, void** hip_stream);
+psdp_status_t psdp_set_seed(psdp_handle_t handle, uint64_t seed);
+psdp_status_t psdp_synchronize(psdp_handle_t handle);
+const char* psdp_last_error(void);
+const char* psdp_status_string(psdp_status_t status);
+psdp_status_t psdp_default_config(psdp_config_t* cfg, int method);
+
+/* Bytes of workspace a projection of size n with cfg needs (ws may then be
+ * passed explicitly; ws == NULL uses the handle's arena). */
+psdp_status_t psdp_workspace_size(const psdp_config_t* cfg, int64_t n, size_t* bytes);
+
+/* Lanczos bound of the last auto-scaled projection (waits for the stream). */
+psdp_status_t psdp_last_bound(psdp_handle_t handle, double* lo, double* up, int* iterations);
+
+/* ---- drop-ins of the reference entry points (in place, FP64, synchronous) ---- */
+psdp_status_t psdp_composite_fp32(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes);
+psdp_status_t psdp_composite_fp32_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                             size_t ws_bytes);
+psdp_status_t psdp_composite_fp16(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes);
+psdp_status_t psdp_composite_fp16_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                             size_t ws_bytes);
+psdp_status_t psdp_composite_fp32_emulated(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                           size_t ws_bytes);
+psdp_status_t psdp_composite_fp32_emulated_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                                      size_t ws_bytes);
+/* lo/up are host pointers, as in the reference (the call blocks). */
+psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_t n, double* lo, double* up,
+                                        int64_t max_iter, double tol);
+
+/* ---- general projections: Y = Pi(X); X == Y allowed; column-major, ld = n ---- */
+psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int64_t n,
+                               const psdp_config_t* cfg, void* ws, size_t ws_bytes);
+psdp_status_t psdp_project_f32(psdp_handle_t h, const float* X, float* Y, int64_t n,
+                               const psdp_config_t* cfg, void* ws, size_t ws_bytes);
+
+/* One symmetric product C = X Y (X, Y symmetric n x n float32, column-major)
+ * with operands rounded to `format`; C receives the float32 result.  Exposes
+ * the filter-chain GEMM kernel on its own. */
+psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float* Y, float* C, int64_t n,
+                                   int format);
+
+/* ---- profiling: HIP events around every launch of the handle, by kind ---- */
+typedef enum psdp_prof_kind {
+  PSDP_PROF_PROD_F32 = 0, /* + psdp_format_t: symmetric products by operand format */
+  PSDP_PROF_LANCZOS = 6,  /* the whole spectral bound                                */
+  PSDP_PROF_PREP = 7,     /* symmetrize/pad + scale/split                            */
+  PSDP_PROF_KINDS = 8
+} psdp_prof_kind_t;
+/* on != 0 starts recording (and clears old records); 0 stops. */
+psdp_status_t psdp_profile_enable(psdp_handle_t h, int on);
+/* Waits for the stream; returns launches and summed milliseconds of a kind. */
+psdp_status_t psdp_profile_summary(psdp_handle_t h, int kind, int* launches, double* total_ms);
+
+/* ---- batched block-diagonal cone: every block projected independently ---- */
+typedef struct psdp_block {
+  const float* X; /* device, column-major n x n */
+  float* Y;       /* device output (may equal X) */
+  int64_t n;
+} psdp_block_t;
+psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
+                                       const psdp_config_t* cfg);
+
+/* ---- EVD projections (rocSOLVER), drop-ins of eig_FP64_psd / eig_FP32_psd ----
+ * eigenvalues: NULL or a caller-provided device array of n doubles that
+ * receives the eigenvalues of the input (ascending) before clamping. */
+psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
+psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSD_PROJECTION_PSDP_H */

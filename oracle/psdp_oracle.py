@@ -1,0 +1,483 @@
+"""CPU oracle for the PSD-projection hot path.  TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker (or, for the CPU
+baseline, as the timed CPU port of the reference's ``eig_FP64_psd``).  The
+library in ``psd_projection_amd`` never imports it and has no CPU fallback.
+
+Everything here restates the reference algorithm in NumPy (float64 unless a
+rounding format is emulated on purpose).  Each function cites the reference
+file:line it follows.  How each function is pinned (golden vectors, closed
+forms, invariants) is listed in PINNING at the bottom; tests/test_oracle.py
+holds the checks.
+
+Reference files (ComputationalRobotics/psd_projection):
+  src/composite_FP32.cu, src/composite_FP16.cu, src/composite_FP32_emulated.cu,
+  src/lanczos.cu, src/eig_FP64_psd.cu, src/eig_FP32_psd.cu, src/utils.cu,
+  test/test_utils.hpp
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# ----------------------------------------------------------------------------
+# Coefficient tables (data, kept identical to the reference on purpose: the
+# method is defined by these numbers).
+# ----------------------------------------------------------------------------
+
+# src/composite_FP32.cu:73-84 ("10 polar express refined"); also used by
+# src/composite_FP32_emulated.cu:78-89.  Stored by the reference as float.
+COEFF_FP32_TABLE = (
+    (8.3119043343, -23.0739115930, 16.4664144722),
+    (4.1439360087, -2.9176674704, 0.5246212487),
+    (4.0257813209, -2.9025002398, 0.5334261214),
+    (3.5118574347, -2.5740236523, 0.5050097282),
+    (2.4398158400, -1.7586675341, 0.4191290613),
+    (1.9779835097, -1.3337358510, 0.3772169049),
+    (1.9559726949, -1.3091355170, 0.3746734515),
+    (1.9282822454, -1.2823649693, 0.3704626545),
+    (1.9220135179, -1.2812524618, 0.3707011753),
+    (1.8942192942, -1.2613293407, 0.3676616051),
+)
+# src/composite_FP32.cu:86,94 : scale factor 1.001 applied to steps 0..7
+FP32_SCALE_FACTOR = 1.001
+FP32_SCALED_STEPS = 8
+
+# src/composite_FP16.cu:93-101 ("polar express with refinement")
+COEFF_FP16_TABLE = (
+    (8.2885332412, -22.5927099246, 15.8201383114),
+    (4.1666196466, -2.9679004036, 0.5307623217),
+    (4.0611848147, -2.9698947955, 0.5492133813),
+    (3.6678301399, -2.7561018955, 0.5421513305),
+    (2.7632556383, -2.0607754898, 0.4695405857),
+    (2.0527445797, -1.4345145882, 0.4070669182),
+    (1.8804816691, -1.2583997294, 0.3779501813),
+)
+# src/composite_FP16.cu:103,111-113 : scale factor 1.01 applied to all steps
+FP16_SCALE_FACTOR = 1.01
+FP16_SCALED_STEPS = 7
+
+
+def step_coefficients(table, scale_factor, n_scaled):
+    """Per-step (a, b, c) exactly as the reference computes them in float32.
+
+    src/composite_FP32.cu:86-99 and src/composite_FP16.cu:103-113:
+    a /= sf; b /= sf*sf*sf; c /= sf*sf*sf*sf*sf, all in float arithmetic
+    (C++ evaluates the products left to right).
+    """
+    f = np.float32
+    sf = f(scale_factor)
+    out = []
+    for i, (a, b, c) in enumerate(table):
+        a, b, c = f(a), f(b), f(c)
+        if i < n_scaled:
+            a = f(a / sf)
+            b = f(b / f(f(sf * sf) * sf))
+            c = f(c / f(f(f(f(sf * sf) * sf) * sf) * sf))
+        out.append((a, b, c))
+    return out
+
+
+def fp32_chain_coefficients():
+    return step_coefficients(COEFF_FP32_TABLE, FP32_SCALE_FACTOR, FP32_SCALED_STEPS)
+
+
+def fp16_chain_coefficients():
+    return step_coefficients(COEFF_FP16_TABLE, FP16_SCALE_FACTOR, FP16_SCALED_STEPS)
+
+
+# ----------------------------------------------------------------------------
+# Scalar transfer function: composite(X) = Q g(L) Q^T in exact arithmetic.
+# ----------------------------------------------------------------------------
+
+def polynomial_chain_scalar(x, coeffs):
+    """P(x) = p_K o ... o p_1 (x), p_i(x) = a x + b x^3 + c x^5, float64.
+
+    Scalar form of the loop src/composite_FP32.cu:89-119 (A2 = A A,
+    A3 = A2 A, A <- a A + b A3 + c A3 A2)."""
+    x = np.asarray(x, dtype=np.float64)
+    for a, b, c in coeffs:
+        x2 = x * x
+        x3 = x2 * x
+        x = float(a) * x + float(b) * x3 + float(c) * (x3 * x2)
+    return x
+
+
+def transfer_function(x, coeffs):
+    """g(x) = x (1 + P(x)) / 2: src/composite_FP32.cu:143-150."""
+    x = np.asarray(x, dtype=np.float64)
+    return x * (1.0 + polynomial_chain_scalar(x, coeffs)) * 0.5
+
+
+# ----------------------------------------------------------------------------
+# Rounding formats used as GEMM operands (emulated on the CPU).
+# ----------------------------------------------------------------------------
+
+def round_f16(x):
+    """float32 -> IEEE binary16 round-to-nearest-even -> float32
+    (src/utils.cu:106-121 uses __float22half2_rn)."""
+    return np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float32)
+
+
+def round_bf16(x):
+    """float32 -> bfloat16 round-to-nearest-even -> float32 (finite inputs)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+F16X3_LO_SCALE = 2048.0  # 2**11: keeps the low FP16 part out of the subnormal range
+
+
+def split_operand(x, fmt):
+    """Return the list of float32 pieces a GEMM operand is stored as."""
+    x = np.asarray(x, dtype=np.float32)
+    if fmt == "f64":
+        return [np.asarray(x, dtype=np.float64)]
+    if fmt == "f32":
+        return [x]
+    if fmt == "f16":
+        return [round_f16(x)]
+    if fmt == "f16x3":
+        hi = round_f16(x)
+        lo = round_f16((x - hi) * np.float32(F16X3_LO_SCALE))
+        return [hi, lo]
+    if fmt == "bf16":
+        return [round_bf16(x)]
+    if fmt == "bf16x3":
+        hi = round_bf16(x)
+        return [hi, round_bf16(x - hi)]
+    if fmt in ("bf16x9", "bf16x6"):
+        hi = round_bf16(x)
+        r = x - hi
+        mid = round_bf16(r)
+        return [hi, mid, round_bf16(r - mid)]
+    raise ValueError(fmt)
+
+
+def emulated_product(x, y, fmt):
+    """x @ y with operands rounded to ``fmt`` pieces and float32 accumulation
+    (products of two 16-bit operands are exact in float32)."""
+    if fmt == "f64":
+        return np.asarray(x, np.float64) @ np.asarray(y, np.float64)
+    px, py = split_operand(x, fmt), split_operand(y, fmt)
+    mm = lambda a, b: (a @ b).astype(np.float32)
+    if fmt in ("f32", "f16", "bf16"):
+        return mm(px[0], py[0])
+    if fmt == "f16x3":
+        cross = mm(px[0], py[1]) + mm(px[1], py[0])
+        return mm(px[0], py[0]) + cross * np.float32(1.0 / F16X3_LO_SCALE)
+    if fmt == "bf16x3":
+        return mm(px[0], py[0]) + (mm(px[0], py[1]) + mm(px[1], py[0]))
+    if fmt == "bf16x6":
+        pairs = [(1, 1), (0, 2), (2, 0), (0, 1), (1, 0), (0, 0)]
+    else:  # bf16x9
+        pairs = [(2, 2), (1, 2), (2, 1), (1, 1), (0, 2), (2, 0), (0, 1), (1, 0), (0, 0)]
+    acc = np.zeros((x.shape[0], y.shape[1]), np.float32)
+    for i, j in pairs:
+        acc = acc + mm(px[i], py[j])
+    return acc
+
+
+def _sym(m):
+    return (m + m.T) * m.dtype.type(0.5)
+
+
+def composite_chain(X, coeffs, step_formats, final_format=None):
+    """Composite polynomial filter chain on an already scaled symmetric X.
+
+    Op order follows src/composite_FP32.cu:89-156 (and the FP16 variant
+    src/composite_FP16.cu:106-233; the emulated variant
+    src/composite_FP32_emulated.cu:93-227):
+        for each step: A2 = A A; A3 = A2 A; A = a A + b A3 + c (A3 A2);
+                       A = (A + A^T)/2
+        A = (I + A)/2; W = X A; W = (W + W^T)/2
+    ``step_formats[i]`` is the operand format of step i's three products;
+    "f64" runs the whole chain in float64 (the exact-arithmetic restatement),
+    any other format keeps the iterate in float32 like the reference.
+    """
+    exact = all(f == "f64" for f in step_formats) and final_format in (None, "f64")
+    dt = np.float64 if exact else np.float32
+    A = np.asarray(X, dtype=dt).copy()
+    Xs = A.copy()
+    for (a, b, c), fmt in zip(coeffs, step_formats):
+        a, b, c = dt(a), dt(b), dt(c)
+        A2 = emulated_product(A, A, fmt).astype(dt)
+        A3 = emulated_product(A2, A, fmt).astype(dt)
+        A = a * A + b * A3
+        A = A + c * emulated_product(A3, A2, fmt).astype(dt)
+        A = _sym(A)
+    n = A.shape[0]
+    A = (A + np.eye(n, dtype=dt)) * dt(0.5)
+    W = emulated_product(Xs, A, final_format or step_formats[-1]).astype(dt)
+    return _sym(W).astype(np.float64)
+
+
+def composite_FP32(X):
+    """src/composite_FP32.cu:15-164 (FP32 chain, 10 steps)."""
+    c = fp32_chain_coefficients()
+    return composite_chain(X, c, ["f32"] * len(c), "f32")
+
+
+def composite_FP16(X):
+    """src/composite_FP16.cu:16-246 (FP16 operands, FP32 accumulate, 7 steps)."""
+    c = fp16_chain_coefficients()
+    return composite_chain(X, c, ["f16"] * len(c), "f16")
+
+
+def composite_FP32_emulated(X):
+    """src/composite_FP32_emulated.cu:20-235 restated with the published BF16x9
+    split product (Henry, Tang, Heinecke, ARITH 2019)."""
+    c = fp32_chain_coefficients()
+    return composite_chain(X, c, ["bf16x9"] * len(c), "bf16x9")
+
+
+def composite_exact(X, table="fp32"):
+    """The chain in float64 arithmetic (same op order, coefficients rounded to
+    float like the reference)."""
+    c = fp32_chain_coefficients() if table == "fp32" else fp16_chain_coefficients()
+    return composite_chain(X, c, ["f64"] * len(c), "f64")
+
+
+# ----------------------------------------------------------------------------
+# Lanczos spectral-norm bound: src/lanczos.cu:41-293
+# ----------------------------------------------------------------------------
+
+def lanczos_two_norm(A, max_iter=20, tol=1e-10, q0=None, return_state=False):
+    """Restatement of approximate_two_norm (src/lanczos.cu:41-293).
+
+    Lanczos on A^T A without reorthogonalisation.  Break rule of
+    src/lanczos.cu:145 (beta_k <= tol*alpha_k and k > 1, alpha_k dropped) and
+    :152-159 (beta_k == 0).  T = tridiag(alpha, beta) (src/lanczos.cu:16-38),
+    theta = largest Ritz value (:231), y = V u (:242).  The residual follows
+    the reference as written: ``cublasDcopy(n, ry, 1, w1, 1)`` at :255 copies
+    ry INTO w1, so r = A y - theta y (not A^T A y - theta y).
+    lo = sqrt(theta) (:264), up = sqrt(theta + ||r||) (:270).  The all-zero
+    path (nb_iter == 0) returns lo = 0, up = 1 (:171-184).
+    ``q0`` replaces the cuRAND start vector (src/utils.cu:201-214); it is
+    normalised as at :103-107.
+    """
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    max_iter = min(int(max_iter), n)
+    if q0 is None:
+        q0 = np.random.default_rng(0).random(n) + 1e-300
+    q = np.array(q0, dtype=np.float64)
+    nq = np.linalg.norm(q)
+    if nq != 0.0:
+        q = q * (1.0 / nq)
+    V = np.zeros((n, max_iter))
+    V[:, 0] = q
+    v_old = np.zeros(n)
+    alpha = np.zeros(max_iter)
+    beta = np.zeros(max_iter)
+    beta_old = 0.0
+    nb = 0
+    for k in range(max_iter):
+        w = A.T @ (A @ q)
+        alpha[k] = q @ w
+        w = w - alpha[k] * q - beta_old * v_old
+        beta[k] = np.linalg.norm(w)
+        if beta[k] <= tol * alpha[k] and k > 1:
+            break
+        v_old = q.copy()
+        if beta[k] == 0.0:
+            break
+        q = w * (1.0 / beta[k])
+        if k < max_iter - 1:
+            V[:, k + 1] = q
+        beta_old = beta[k]
+        nb += 1
+    if nb == 0:
+        res = (0.0, 1.0)
+        if return_state:
+            return res + (dict(nb=0, alpha=alpha, beta=beta, theta=None),)
+        return res
+    T = np.diag(alpha[:nb]) + np.diag(beta[: nb - 1], 1) + np.diag(beta[: nb - 1], -1)
+    evals, evecs = np.linalg.eigh(T)
+    theta = evals[-1]
+    u = evecs[:, -1]
+    y = V[:, :nb] @ u
+    r = A @ y - theta * y
+    lo = np.sqrt(theta)
+    up = np.sqrt(theta + np.linalg.norm(r))
+    if return_state:
+        return lo, up, dict(nb=nb, alpha=alpha[:nb], beta=beta[:nb], theta=theta, T=T)
+    return lo, up
+
+
+def composite_auto_scale(X, method="fp32", q0=None, max_iter=20, tol=1e-10):
+    """composite_*_auto_scale (src/composite_FP32.cu:166-191,
+    src/composite_FP16.cu:248-274, src/composite_FP32_emulated.cu:237-262):
+    up from Lanczos, s = up if up > 0 else 1, Y = s * chain(X / s)."""
+    _, up = lanczos_two_norm(X, max_iter, tol, q0)
+    s = up if up > 0.0 else 1.0
+    Xs = np.asarray(X, np.float64) * (1.0 / s)
+    f = {"fp32": composite_FP32, "fp16": composite_FP16,
+         "fp32_emulated": composite_FP32_emulated,
+         "exact": composite_exact}[method]
+    return f(Xs) * s
+
+
+# ----------------------------------------------------------------------------
+# EVD projection (oracle and CPU-baseline method): src/eig_FP64_psd.cu:24-85
+# ----------------------------------------------------------------------------
+
+def eig_FP64_psd(X, return_eigenvalues=False):
+    """dsyevd (jobz=V, uplo=U) -> clamp at 0 -> scale columns -> V diag V^T.
+    LAPACK dsyevd via scipy (driver 'evd', upper triangle)."""
+    import scipy.linalg as sla
+    X = np.asarray(X, dtype=np.float64)
+    w, V = sla.eigh(X, lower=False, driver="evd", check_finite=False)
+    wp = np.maximum(w, 0.0)
+    Y = (V * wp) @ V.T
+    if return_eigenvalues:
+        return Y, w
+    return Y
+
+
+def eig_FP32_psd(X, return_eigenvalues=False):
+    """src/eig_FP32_psd.cu:23-96: FP64 -> FP32, ssyevd, clamp, FP32 V diag V^T."""
+    import scipy.linalg as sla
+    Xs = np.asarray(X, dtype=np.float32)
+    w, V = sla.eigh(Xs, lower=False, driver="evd", check_finite=False)
+    w = w.astype(np.float32)
+    Y = ((V * np.maximum(w, np.float32(0))) @ V.T).astype(np.float64)
+    if return_eigenvalues:
+        return Y, w.astype(np.float64)
+    return Y
+
+
+def closed_form_projection(Q, lam):
+    """Pi(X) = Q max(L,0) Q^T for X = Q diag(lam) Q^T."""
+    Q = np.asarray(Q, np.float64)
+    return (Q * np.maximum(np.asarray(lam, np.float64), 0.0)) @ Q.T
+
+
+def rel_fro_error(Y, P):
+    Y = np.asarray(Y, np.float64)
+    P = np.asarray(P, np.float64)
+    return float(np.linalg.norm(Y - P) / np.linalg.norm(P))
+
+
+# ----------------------------------------------------------------------------
+# Input generators
+# ----------------------------------------------------------------------------
+
+class GlibcRandom:
+    """glibc random()/rand() TYPE_3 additive-feedback generator (the published
+    glibc random_r algorithm: r[i] = r[i-3] + r[i-31] mod 2^32, output r >> 1,
+    seeded by r[i] = 16807 r[i-1] mod (2^31-1), 310 warm-up outputs)."""
+
+    RAND_MAX = 2147483647
+
+    def __init__(self, seed=1):
+        r = [0] * 34
+        r[0] = seed if seed != 0 else 1
+        for i in range(1, 31):
+            hi, lo = divmod(r[i - 1], 127773)
+            word = 16807 * lo - 2836 * hi
+            if word < 0:
+                word += 2147483647
+            r[i] = word
+        for i in range(31, 34):
+            r[i] = r[i - 31]
+        self._r = r
+        self._warm = 310
+
+    def take(self, count):
+        """Return ``count`` successive rand() outputs as int64 array."""
+        total = self._warm + count
+        r = np.zeros(34 + total, dtype=np.uint64)
+        r[:34] = np.array(self._r, dtype=np.uint64)
+        mask = np.uint64(0xFFFFFFFF)
+        # r[i] = r[i-31] + r[i-3]; vectorised in chunks of 3 (the shortest lag)
+        i = 34
+        end = 34 + total
+        while i < end:
+            j = min(i + 3, end)
+            r[i:j] = (r[i - 31:j - 31] + r[i - 3:j - 3]) & mask
+            i = j
+        out = (r[34 + self._warm:] >> np.uint64(1)).astype(np.int64)
+        self._r = [int(v) for v in r[-34:]]
+        self._warm = 0
+        return out
+
+
+def generate_and_project(n, radius=1.0):
+    """test/test_utils.hpp:20-164 (generateAndProject) on the host.
+
+    srand(1); M_ij = rand()/RAND_MAX - 0.5 (filled in memory order, i.e.
+    column-major); Q from Householder QR (LAPACK dgeqrf/dorgqr, the same
+    convention cuSOLVER follows); lambda_i = -r + 2 i r/(n-1); A = (Q L) Q^T,
+    symmetrised; oracle = FP64 EVD projection.  Returns (A, A_psd, Q, lam)."""
+    g = GlibcRandom(1)
+    vals = g.take(n * n).astype(np.float64) / GlibcRandom.RAND_MAX - 0.5
+    M = vals.reshape(n, n, order="F")
+    Q, _ = np.linalg.qr(M, mode="reduced")
+    lam = -radius + 2.0 * np.arange(n) * radius / float(n - 1)
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    return A, eig_FP64_psd(A), Q, lam
+
+
+def haar_orthogonal(n, seed):
+    """Haar-distributed Q: QR of a seeded N(0,1) matrix with the sign fix
+    Q <- Q diag(sign(diag R))."""
+    G = np.random.default_rng(seed).standard_normal((n, n))
+    Q, R = np.linalg.qr(G)
+    return Q * np.sign(np.diag(R))
+
+
+def config1_input(n=512):
+    """Config 1: n=512, Q from QR of N(0,1) seed 1, lambda ~ U[-1,1] seed 1001."""
+    Q = haar_orthogonal(n, 1)
+    lam = np.random.default_rng(1001).uniform(-1.0, 1.0, n)
+    X = (Q * lam) @ Q.T
+    X = 0.5 * (X + X.T)
+    return X, Q, lam
+
+
+def clustered_spectrum(n, seed, kind):
+    """Spectra of configs 2/3 (scaled by the same rule at any n)."""
+    rng = np.random.default_rng(seed + 100)
+    if kind == "cfg2":
+        lam = rng.standard_normal(n)
+        lam *= 10.0 / np.max(np.abs(lam))
+        idx = rng.choice(n, int(round(0.3 * n)), replace=False)
+        lam[idx] = rng.uniform(-1e-2, 1e-2, idx.size)
+    elif kind == "cfg3":
+        lam = rng.uniform(-5.0, 5.0, n)
+        idx = rng.choice(n, int(round(0.1 * n)), replace=False)
+        lam[idx] = rng.uniform(-1e-3, 1e-3, idx.size)
+    elif kind == "uniform":
+        lam = rng.uniform(-1.0, 1.0, n)
+    elif kind == "normal":
+        lam = rng.standard_normal(n)
+    else:
+        raise ValueError(kind)
+    return lam
+
+
+def spectral_input(n, seed, kind):
+    Q = haar_orthogonal(n, seed)
+    lam = clustered_spectrum(n, seed, kind)
+    X = (Q * lam) @ Q.T
+    X = 0.5 * (X + X.T)
+    return X, Q, lam
+
+
+# How each oracle function is pinned (checked in tests/test_oracle.py).
+PINNING = {
+    "step_coefficients": "float32 re-derivation + transfer-function KATs",
+    "transfer_function": "max|g-ReLU| on [-1,1] (4.48e-6 FP32 table, 4.65e-5 FP16 table), "
+                         "oddness of P, g(x)=x for x in [1e-2,1], g(x)=0 for x in [-1,-1e-2]",
+    "composite_chain(f64)": "equals Q g(L) Q^T to 1e-12 (closed form); reference KATs "
+                            "(3x3 identity map, diag(1,-2,3,-4,5))",
+    "emulated_product": "f32 matches float64 within 1e-6; f16/bf16 rounding matches numpy "
+                        "float16 / bit patterns; x3/x9 splits reconstruct the operand",
+    "lanczos_two_norm": "5x5 KAT (7.089012550562305) bracketed; up formula vs direct "
+                        "recomputation from T; lo <= ||A|| on random inputs; zero matrix -> (0,1)",
+    "eig_FP64_psd": "diag KAT, PSD fixed point, closed form Q max(L,0) Q^T",
+    "GlibcRandom": "first outputs after srand(1) (1804289383, 846930886, ...) and libc rand()",
+}

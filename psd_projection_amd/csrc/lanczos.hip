@@ -1,0 +1,346 @@
+// Device-resident Lanczos spectral-norm bound (approximate_two_norm).
+//
+// Restates src/lanczos.cu:41-293 with every scalar kept in device memory so
+// that the whole bound is a fixed sequence of launches (no host round trips,
+// graph capturable):
+//   * Lanczos on A^T A = A A (A symmetric), no reorthogonalisation
+//     (src/lanczos.cu:116-169), the break rule of :145 and :152-159 evaluated
+//     on the device; after a break the remaining launches are no-ops;
+//   * T = tridiag(alpha, beta) (src/lanczos.cu:16-38), its largest eigenpair
+//     by a one-wave cyclic Jacobi solver in FP64 (replaces cusolverDnDsyevd
+//     at :206-214; T is built from zeros, unlike the unzeroed buffer there);
+//   * y = V u (:242), r = A y - theta y (the reference's :255 copies ry into
+//     w1, so its residual is A y - theta y), lo = sqrt(theta) (:264),
+//     up = sqrt(theta + ||r||) (:270); nb_iter == 0 gives lo = 0, up = 1
+//     (:171-184).
+// The matrix-vector products are a bandwidth-bound GEMV (one wave streams
+// RPW rows with 16-byte loads, FP64 accumulation, wave-shuffle reduction);
+// vector updates run in one 1024-thread workgroup per matrix.
+#include "psdp_device.h"
+
+namespace psdp {
+
+// ------------------------------------------------------------------ GEMV
+// y[r] = sum_k A[r*ld + k] x[k] for r < m (A symmetric: a row of the
+// row-major view is a column of the column-major matrix).
+template <typename TA, int VEC, int RPW>
+__global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A, int64_t ld, int m,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y,
+                                                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & 63;
+  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+  if (r0 >= m) return;
+  double acc[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) acc[r] = 0.0;
+  for (int k = lane * VEC; k < m; k += 64 * VEC) {
+    double xv[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xv[e] = x[k + e];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      if (r0 + r < m) {
+        const TA* row = A + (int64_t)(r0 + r) * ld + k;
+        TA av[VEC];
+        if constexpr (VEC * sizeof(TA) == 16) {
+          *(uint4*)av = *(const uint4*)row;
+        } else {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) av[e] = row[e];
+        }
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[r] = fma((double)av[e], xv[e], acc[r]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const double s = wave_sum(acc[r]);
+    if (lane == 0 && r0 + r < m) y[r0 + r] = s;
+  }
+}
+
+// ------------------------------------------------------------------ start
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// q_i ~ U(0,1] for i < n (zero in the padding), q <- q/||q||, V(:,0) = q,
+// vold = 0 (src/lanczos.cu:69,101-112).  The start vector is a seeded
+// counter-based stream instead of cuRAND XORWOW with a clock seed.
+__global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, LanczosVecs v, int n, int m,
+                                                            uint64_t seed) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double qi = 0.0;
+    if (i < n) qi = ((double)(splitmix64(seed ^ (0x632be59bd9b4e019ull * (uint64_t)(i + 1))) >> 11) + 1.0) *
+                    (1.0 / 9007199254740992.0);
+    v.q[i] = qi;
+    s += qi * qi;
+  }
+  const double nrm = sqrt(block_sum(s, red));
+  const double inv = nrm != 0.0 ? 1.0 / nrm : 1.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double qi = v.q[i] * inv;
+    v.q[i] = qi;
+    v.V[i] = qi;
+    v.vold[i] = 0.0;
+  }
+  if (threadIdx.x == 0) {
+    st->beta_old = 0.0;
+    st->nb = 0;
+    st->done = 0;
+    st->skip_ritz = 0;
+    st->nonfinite = 0;
+    st->theta = 0.0;
+    st->rnorm = 0.0;
+    st->lo = 0.0;
+    st->up = 1.0;
+    st->scale = 1.0;
+    st->inv_scale = 1.0;
+  }
+}
+
+// One Lanczos iteration after w = A (A q) (src/lanczos.cu:130-168).
+__global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int m,
+                                                            int max_iter, double tol) {
+  __shared__ double red[16];
+  if (st->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) s += v.q[i] * v.w[i];
+  const double alpha = block_sum(s, red);
+  const double mbeta_old = -st->beta_old;
+  s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double wi = fma(-alpha, v.q[i], v.w[i]);
+    wi = fma(mbeta_old, v.vold[i], wi);
+    v.w[i] = wi;
+    s += wi * wi;
+  }
+  const double beta = sqrt(block_sum(s, red));
+  const bool bad = !isfinite(alpha) || !isfinite(beta);
+  const bool stop = bad || (beta <= tol * alpha && k > 1) || beta == 0.0;
+  if (!stop) {
+    const double inv = 1.0 / beta;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const double qi = v.w[i] * inv;
+      v.vold[i] = v.q[i];
+      v.q[i] = qi;
+      if (k < max_iter - 1) v.V[(int64_t)(k + 1) * m + i] = qi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->alpha[k] = alpha;
+    st->beta[k] = beta;
+    if (bad) st->nonfinite = 1;
+    if (stop) {
+      st->done = 1;
+    } else {
+      st->beta_old = beta;
+      st->nb = k + 1;
+      if (k == max_iter - 1) st->done = 1;
+    }
+  }
+}
+
+// Largest eigenpair of T = tridiag(alpha[0..nb), beta[0..nb-1)) by cyclic
+// Jacobi rotations in FP64, one wave (lane k owns row/column k).
+__global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) {
+  __shared__ double T[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
+  __shared__ double Z[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
+  const int lane = threadIdx.x;
+  const int nb = st->nb;
+  if (nb == 0 || st->nonfinite) {
+    if (lane == 0) {
+      st->skip_ritz = 1;
+      if (st->nonfinite) {
+        st->lo = st->up = __builtin_nan("");
+        st->scale = st->inv_scale = __builtin_nan("");
+      } else {
+        st->lo = 0.0;
+        st->up = 1.0;
+        st->scale = 1.0;
+        st->inv_scale = 1.0;
+      }
+    }
+    return;
+  }
+  for (int c = 0; c < nb; ++c) {
+    if (lane < nb) {
+      double t = 0.0;
+      if (lane == c) t = st->alpha[c];
+      else if (lane == c + 1) t = st->beta[c];
+      else if (lane + 1 == c) t = st->beta[lane];
+      T[lane][c] = t;
+      Z[lane][c] = lane == c ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    if (lane < nb) {
+      for (int c = 0; c < nb; ++c) {
+        const double t = T[lane][c];
+        if (c == lane) dia += t * t;
+        else off += t * t;
+      }
+    }
+    off = wave_sum(off);
+    dia = wave_sum(dia);
+    if (off <= 1e-30 * dia || off == 0.0) break;
+    for (int p = 0; p < nb - 1; ++p) {
+      for (int q = p + 1; q < nb; ++q) {
+        const double apq = T[p][q];
+        const double app = T[p][p], aqq = T[q][q];
+        if (fabs(apq) <= 1e-300 || fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq))) {
+          __syncthreads();
+          if (lane == 0) { T[p][q] = 0.0; T[q][p] = 0.0; }
+          __syncthreads();
+          continue;
+        }
+        const double th = (aqq - app) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0);
+        const double s = t * c;
+        double kp = 0.0, kq = 0.0, zp = 0.0, zq = 0.0;
+        if (lane < nb) {
+          kp = T[lane][p];
+          kq = T[lane][q];
+          zp = Z[lane][p];
+          zq = Z[lane][q];
+        }
+        __syncthreads();
+        if (lane < nb) {
+          if (lane != p && lane != q) {
+            const double np_ = c * kp - s * kq;
+            const double nq_ = s * kp + c * kq;
+            T[lane][p] = np_;
+            T[p][lane] = np_;
+            T[lane][q] = nq_;
+            T[q][lane] = nq_;
+          }
+          Z[lane][p] = c * zp - s * zq;
+          Z[lane][q] = s * zp + c * zq;
+        }
+        if (lane == 0) {
+          T[p][p] = app - t * apq;
+          T[q][q] = aqq + t * apq;
+          T[p][q] = 0.0;
+          T[q][p] = 0.0;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // largest eigenvalue and its vector
+  double best = -1e308;
+  int bi = 0;
+  for (int c = 0; c < nb; ++c) {
+    const double d = T[c][c];
+    if (d > best) {
+      best = d;
+      bi = c;
+    }
+  }
+  if (lane < nb) st->u[lane] = Z[lane][bi];
+  if (lane == 0) st->theta = best;
+}
+
+// y = V(:, 0:nb) u  (src/lanczos.cu:242)
+__global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st, LanczosVecs v, int m) {
+  if (st->skip_ritz) return;
+  const int nb = st->nb;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int j = 0; j < nb; ++j) s = fma(v.V[(int64_t)j * m + i], st->u[j], s);
+  v.y[i] = s;
+}
+
+// r = A y - theta y; lo = sqrt(theta); up = sqrt(theta + ||r||)
+// (src/lanczos.cu:258-270); s = up > 0 ? up : 1 (src/composite_FP32.cu:182).
+__global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, LanczosVecs v, int m,
+                                                             double margin) {
+  __shared__ double red[16];
+  if (st->skip_ritz) return;
+  const double th = st->theta;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double r = fma(-th, v.y[i], v.ay[i]);
+    s += r * r;
+  }
+  const double rn = sqrt(block_sum(s, red));
+  if (threadIdx.x == 0) {
+    const double up = sqrt(th + rn);
+    st->rnorm = rn;
+    st->lo = sqrt(th);
+    st->up = up;
+    const double sc = up > 0.0 ? up * margin : 1.0;
+    st->scale = sc;
+    st->inv_scale = 1.0 / sc;
+    if (!isfinite(up)) st->nonfinite = 1;
+  }
+}
+
+// Fixed scale (no bound): used by the non-auto-scaled entry points.
+__global__ void set_scale_kernel(LanczosState* st, double scale) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->scale = scale;
+    st->inv_scale = 1.0 / scale;
+    st->nonfinite = 0;
+  }
+}
+
+template <typename TA>
+static void launch_gemv(const TA* A, int64_t ld, int m, const double* x, double* y, const int* skip,
+                        hipStream_t s) {
+  constexpr int RPW = 4;
+  const int rows_per_block = 4 * RPW;
+  const int grid = (m + rows_per_block - 1) / rows_per_block;
+  constexpr int VEC = 16 / (int)sizeof(TA);
+  const bool aligned = (m % VEC == 0) && (ld % VEC == 0) && (((uintptr_t)A) % 16 == 0);
+  if (aligned)
+    hipLaunchKernelGGL((gemv_sym_kernel<TA, VEC, RPW>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
+  else
+    hipLaunchKernelGGL((gemv_sym_kernel<TA, 1, RPW>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
+}
+
+// Enqueues the whole bound.  A: symmetric m x m (row-major view = column-major),
+// logical size n <= m (entries beyond n are zero).  Vectors have length m.
+hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
+                          uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, s, st, v, n, m, seed);
+  for (int k = 0; k < max_iter; ++k) {
+    if (a_f64) {
+      launch_gemv((const double*)A, ld, m, v.q, v.w1, &st->done, s);
+      launch_gemv((const double*)A, ld, m, v.w1, v.w, &st->done, s);
+    } else {
+      launch_gemv((const float*)A, ld, m, v.q, v.w1, &st->done, s);
+      launch_gemv((const float*)A, ld, m, v.w1, v.w, &st->done, s);
+    }
+    hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, s, st, v, k, m, max_iter, tol);
+  }
+  hipLaunchKernelGGL(lanczos_ritz_kernel, dim3(1), dim3(64), 0, s, st);
+  hipLaunchKernelGGL(ritz_vector_kernel, dim3((m + 255) / 256), dim3(256), 0, s, st, v, m);
+  if (a_f64)
+    launch_gemv((const double*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
+  else
+    launch_gemv((const float*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
+  hipLaunchKernelGGL(lanczos_bound_kernel, dim3(1), dim3(1024), 0, s, st, v, m, margin);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s) {
+  hipLaunchKernelGGL(set_scale_kernel, dim3(1), dim3(64), 0, s, st, scale);
+  return hipGetLastError();
+}
+
+}  // namespace psdp

@@ -1,0 +1,792 @@
+// Host driver and C ABI of libpsdp.
+//
+// A projection is a fixed sequence of launches on the handle's stream, with
+// every scalar (the Lanczos bound and the scale) kept in device memory:
+//   sym_pad (X -> symmetric float32, zero padded)
+//   Lanczos bound on that matrix (auto-scaled entry points only)
+//   scale_split (A0 = X/s, operand pieces of step 0 and of the final product)
+//   3 symmetric products per filter step (fused polynomial epilogue)
+//   final product W = X_s S with the output epilogue (x s, f64/f32, n x n)
+// Workspace comes from the caller (ws, ws_bytes) or from the handle's arena,
+// which grows on demand and is reused by later calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/psd_projection/psdp.h"
+#include "psdp_internal.h"
+
+namespace psdp {
+hipError_t launch_sym_product(int fmt, int sub, const ProdDesc& d, const TileRef* tiles, int ntiles,
+                              hipStream_t s);
+hipError_t launch_sym_product_grouped(int fmt, int sub, const ProdDesc* descs, const TileRef* tiles,
+                                      int ntiles, int grid, hipStream_t s);
+hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
+                          hipStream_t st);
+hipError_t launch_scale_split(float* a32, int64_t count, const double* inv, int fmt_a, uint16_t* const ap[3],
+                              int fmt_x, float* x32, uint16_t* const xp[3], hipStream_t st);
+hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
+                          uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
+                          hipStream_t s);
+hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s);
+}  // namespace psdp
+
+using namespace psdp;
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_last_error;
+
+static psdp_status_t fail(psdp_status_t st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+#define HIPCHK(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(PSDP_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                     \
+  } while (0)
+
+#define STCHK(call)                        \
+  do {                                     \
+    psdp_status_t s_ = (call);             \
+    if (s_ != PSDP_OK) return s_;          \
+  } while (0)
+
+// ---------------------------------------------------------------- tables
+// src/composite_FP32.cu:73-84 (also src/composite_FP32_emulated.cu:78-89)
+static const double kTableFP32[10][3] = {
+    {8.3119043343, -23.0739115930, 16.4664144722}, {4.1439360087, -2.9176674704, 0.5246212487},
+    {4.0257813209, -2.9025002398, 0.5334261214},   {3.5118574347, -2.5740236523, 0.5050097282},
+    {2.4398158400, -1.7586675341, 0.4191290613},   {1.9779835097, -1.3337358510, 0.3772169049},
+    {1.9559726949, -1.3091355170, 0.3746734515},   {1.9282822454, -1.2823649693, 0.3704626545},
+    {1.9220135179, -1.2812524618, 0.3707011753},   {1.8942192942, -1.2613293407, 0.3676616051}};
+// src/composite_FP16.cu:93-101
+static const double kTableFP16[7][3] = {
+    {8.2885332412, -22.5927099246, 15.8201383114}, {4.1666196466, -2.9679004036, 0.5307623217},
+    {4.0611848147, -2.9698947955, 0.5492133813},   {3.6678301399, -2.7561018955, 0.5421513305},
+    {2.7632556383, -2.0607754898, 0.4695405857},   {2.0527445797, -1.4345145882, 0.4070669182},
+    {1.8804816691, -1.2583997294, 0.3779501813}};
+
+struct Schedule {
+  int nsteps = 0;
+  float a[PSDP_MAX_STEPS], b[PSDP_MAX_STEPS], c[PSDP_MAX_STEPS];
+  int fmt[PSDP_MAX_STEPS];
+  int final_fmt = FMT_F32;
+};
+
+// Coefficients as the reference computes them, in float arithmetic:
+// a /= sf; b /= sf*sf*sf; c /= sf*sf*sf*sf*sf on the scaled steps
+// (src/composite_FP32.cu:86-99, src/composite_FP16.cu:103-113).
+static void fill_table(int table, Schedule& s) {
+  const double(*t)[3] = table == PSDP_TABLE_FP16 ? kTableFP16 : kTableFP32;
+  const int steps = table == PSDP_TABLE_FP16 ? 7 : 10;
+  const int nscaled = table == PSDP_TABLE_FP16 ? 7 : 8;
+  const float sf = table == PSDP_TABLE_FP16 ? 1.01f : 1.001f;
+  s.nsteps = steps;
+  for (int i = 0; i < steps; ++i) {
+    float a = (float)t[i][0], b = (float)t[i][1], c = (float)t[i][2];
+    if (i < nscaled) {
+      const float f = sf;
+      a /= f;
+      b /= f * f * f;
+      c /= f * f * f * f * f;
+    }
+    s.a[i] = a;
+    s.b[i] = b;
+    s.c[i] = c;
+  }
+}
+
+static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
+  switch (cfg->method) {
+    case PSDP_METHOD_COMPOSITE_FP32:
+      fill_table(PSDP_TABLE_FP32, s);
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = FMT_F32;
+      s.final_fmt = FMT_F32;
+      break;
+    case PSDP_METHOD_COMPOSITE_FP16:
+      fill_table(PSDP_TABLE_FP16, s);
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = FMT_F16;
+      s.final_fmt = FMT_F16;
+      break;
+    case PSDP_METHOD_COMPOSITE_FP32_EMULATED:
+      fill_table(PSDP_TABLE_FP32, s);
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = FMT_BF16X9;
+      s.final_fmt = FMT_BF16X9;
+      break;
+    case PSDP_METHOD_MIXED:
+      fill_table(PSDP_TABLE_FP16, s);
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 3 ? FMT_F16X3 : FMT_F16;
+      s.final_fmt = FMT_F16;
+      break;
+    case PSDP_METHOD_CUSTOM:
+      if (cfg->table != PSDP_TABLE_FP32 && cfg->table != PSDP_TABLE_FP16)
+        return fail(PSDP_ERR_INVALID_ARG, "unknown coefficient table %d", cfg->table);
+      fill_table(cfg->table, s);
+      for (int i = 0; i < s.nsteps; ++i) {
+        if (cfg->step_format[i] < 0 || cfg->step_format[i] >= FMT_COUNT)
+          return fail(PSDP_ERR_INVALID_ARG, "step %d: unknown format %d", i, cfg->step_format[i]);
+        s.fmt[i] = cfg->step_format[i];
+      }
+      if (cfg->final_format < 0 || cfg->final_format >= FMT_COUNT)
+        return fail(PSDP_ERR_INVALID_ARG, "unknown final format %d", cfg->final_format);
+      s.final_fmt = cfg->final_format;
+      break;
+    default:
+      return fail(PSDP_ERR_INVALID_ARG, "unknown method %d", cfg->method);
+  }
+  return PSDP_OK;
+}
+
+static int pieces_of(int fmt) {
+  return fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : (fmt == FMT_F32 ? 0 : 1));
+}
+
+// ---------------------------------------------------------------- handle
+struct TileList {
+  TileRef* dev = nullptr;
+  int count = 0;
+};
+
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
+struct psdp_handle_st {
+  bool profiling = false;
+  std::vector<ProfRec> prof;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  LanczosState* st = nullptr;  // device
+  uint64_t seed = 0x5eed5eedULL;
+  std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
+  std::mutex mu;
+};
+
+// Upper-triangle tiles of a T x T grid, grouped into G x G super-tiles so that
+// tiles running together share operand panels, then interleaved so that the
+// hardware's round-robin dispatch gives each of the 8 XCDs one contiguous
+// chunk of that order (its L2 then serves neighbouring panels).
+static std::vector<TileRef> make_tile_order(int T) {
+  const int G = 4;
+  std::vector<TileRef> ord;
+  ord.reserve((size_t)T * (T + 1) / 2);
+  for (int si = 0; si < T; si += G)
+    for (int sj = si; sj < T; sj += G)
+      for (int i = si; i < std::min(si + G, T); ++i)
+        for (int j = std::max(sj, i); j < std::min(sj + G, T); ++j) ord.push_back(TileRef{0, i, j, 0});
+  const int L = (int)ord.size();
+  const int XCD = 8;
+  const int C = (L + XCD - 1) / XCD;
+  std::vector<TileRef> out((size_t)C * XCD, TileRef{0, -1, -1, 0});
+  for (int w = 0; w < C * XCD; ++w) {
+    const int src = (w % XCD) * C + w / XCD;
+    if (src < L) out[w] = ord[src];
+  }
+  return out;
+}
+
+static psdp_status_t get_tiles(psdp_handle_t h, int T, int bt, TileList& tl) {
+  auto key = std::make_pair(T, bt);
+  auto it = h->tiles.find(key);
+  if (it != h->tiles.end()) {
+    tl = it->second;
+    return PSDP_OK;
+  }
+  std::vector<TileRef> v = make_tile_order(T);
+  TileList n;
+  n.count = (int)v.size();
+  if (hipMalloc(&n.dev, v.size() * sizeof(TileRef)) != hipSuccess)
+    return fail(PSDP_ERR_NOMEM, "tile list allocation failed");
+  HIPCHK(hipMemcpy(n.dev, v.data(), v.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  h->tiles[key] = n;
+  tl = n;
+  return PSDP_OK;
+}
+
+static psdp_status_t arena_reserve(psdp_handle_t h, size_t bytes, void** out) {
+  if (bytes > h->arena_bytes) {
+    if (h->arena) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipFree(h->arena));
+      h->arena = nullptr;
+      h->arena_bytes = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PSDP_ERR_NOMEM, "workspace allocation of %zu bytes failed", bytes);
+    }
+    h->arena = p;
+    h->arena_bytes = bytes;
+  }
+  *out = h->arena;
+  return PSDP_OK;
+}
+
+// Brackets one launch with events when profiling is on.
+struct ProfScope {
+  psdp_handle_t h;
+  ProfRec rec;
+  bool on;
+  ProfScope(psdp_handle_t hh, int kind) : h(hh), on(hh->profiling) {
+    if (!on) return;
+    rec.kind = kind;
+    if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(rec.a, h->stream);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(rec.b, h->stream);
+    h->prof.push_back(rec);
+  }
+};
+
+// ---------------------------------------------------------------- plan
+struct Plan {
+  int n = 0, npad = 0, sub = 4, bt = 128;
+  int64_t N = 0;
+  Schedule sch;
+  int lanczos_iters = 20;
+  size_t off_a32 = 0, off_p3 = 0, off_p2 = 0, off_x32 = 0;
+  size_t off_ap[3] = {0, 0, 0}, off_p2p[3] = {0, 0, 0}, off_p3p[3] = {0, 0, 0}, off_xp[3] = {0, 0, 0};
+  size_t off_vec = 0;
+  size_t total = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int choose_sub(int64_t n) { return n > 1024 ? 4 : 2; }
+
+static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
+  if (n <= 0) return fail(PSDP_ERR_INVALID_ARG, "matrix size must be positive (got %lld)", (long long)n);
+  if (n > (1 << 20)) return fail(PSDP_ERR_INVALID_ARG, "matrix size %lld too large", (long long)n);
+  STCHK(build_schedule(cfg, p.sch));
+  p.lanczos_iters = cfg->lanczos_iters > 0 ? cfg->lanczos_iters : 20;
+  if (p.lanczos_iters > LANCZOS_MAX_ITER)
+    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
+  p.n = (int)n;
+  p.sub = choose_sub(n);
+  p.bt = 32 * p.sub;
+  p.npad = (int)((n + p.bt - 1) / p.bt * p.bt);
+  p.N = (int64_t)p.npad * p.npad;
+  bool any_f32 = false;
+  int pa = pieces_of(p.sch.final_fmt), pp = 0;
+  for (int i = 0; i < p.sch.nsteps; ++i) {
+    any_f32 |= p.sch.fmt[i] == FMT_F32;
+    pa = std::max(pa, pieces_of(p.sch.fmt[i]));
+    pp = std::max(pp, pieces_of(p.sch.fmt[i]));
+  }
+  const int px = pieces_of(p.sch.final_fmt);
+  const size_t f32b = align256((size_t)p.N * 4), u16b = align256((size_t)p.N * 2);
+  size_t off = 0;
+  p.off_a32 = off, off += f32b;
+  p.off_p3 = off, off += f32b;
+  if (any_f32) p.off_p2 = off, off += f32b;
+  if (p.sch.final_fmt == FMT_F32) p.off_x32 = off, off += f32b;
+  for (int i = 0; i < pa; ++i) p.off_ap[i] = off, off += u16b;
+  for (int i = 0; i < pp; ++i) p.off_p2p[i] = off, off += u16b;
+  for (int i = 0; i < pp; ++i) p.off_p3p[i] = off, off += u16b;
+  for (int i = 0; i < px; ++i) p.off_xp[i] = off, off += u16b;
+  p.off_vec = off;
+  off += align256((size_t)p.npad * 8) * 6 + align256((size_t)p.npad * 8 * p.lanczos_iters);
+  p.total = off;
+  return PSDP_OK;
+}
+
+struct Bufs {
+  float *a32, *p3, *p2, *x32;
+  uint16_t *ap[3], *p2p[3], *p3p[3], *xp[3];
+  LanczosVecs v;
+};
+
+static Bufs carve(const Plan& p, char* base) {
+  Bufs b;
+  b.a32 = (float*)(base + p.off_a32);
+  b.p3 = (float*)(base + p.off_p3);
+  b.p2 = p.off_p2 ? (float*)(base + p.off_p2) : nullptr;
+  b.x32 = p.off_x32 ? (float*)(base + p.off_x32) : nullptr;
+  for (int i = 0; i < 3; ++i) {
+    b.ap[i] = p.off_ap[i] ? (uint16_t*)(base + p.off_ap[i]) : nullptr;
+    b.p2p[i] = p.off_p2p[i] ? (uint16_t*)(base + p.off_p2p[i]) : nullptr;
+    b.p3p[i] = p.off_p3p[i] ? (uint16_t*)(base + p.off_p3p[i]) : nullptr;
+    b.xp[i] = p.off_xp[i] ? (uint16_t*)(base + p.off_xp[i]) : nullptr;
+  }
+  const size_t vb = align256((size_t)p.npad * 8);
+  char* v = base + p.off_vec;
+  b.v.q = (double*)(v + 0 * vb);
+  b.v.w = (double*)(v + 1 * vb);
+  b.v.w1 = (double*)(v + 2 * vb);
+  b.v.vold = (double*)(v + 3 * vb);
+  b.v.y = (double*)(v + 4 * vb);
+  b.v.ay = (double*)(v + 5 * vb);
+  b.v.V = (double*)(v + 6 * vb);
+  return b;
+}
+
+static void set_operand(const void* (&dst)[3], int fmt, const float* m32, uint16_t* const pieces[3]) {
+  if (fmt == FMT_F32) {
+    dst[0] = m32;
+    dst[1] = dst[2] = nullptr;
+  } else {
+    for (int i = 0; i < 3; ++i) dst[i] = pieces[i];
+  }
+}
+
+static ProdDesc base_desc(const Plan& p) {
+  ProdDesc d;
+  memset(&d, 0, sizeof d);
+  d.ld = p.npad;
+  d.npad = p.npad;
+  d.n = p.n;
+  return d;
+}
+
+// Enqueues the whole projection.
+static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
+                                        void* Y, int y_f64, const psdp_config_t* cfg) {
+  hipStream_t s = h->stream;
+  TileList tl;
+  STCHK(get_tiles(h, p.npad / p.bt, p.bt, tl));
+  const Schedule& sch = p.sch;
+
+  {
+    ProfScope ps(h, PSDP_PROF_PREP);
+    HIPCHK(launch_sym_pad(X, x_f64, p.n, p.n, b.a32, p.npad, p.npad, s));
+  }
+  if (cfg->auto_scale) {
+    const double tol = cfg->lanczos_tol > 0 ? cfg->lanczos_tol : 1e-10;
+    const double margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
+    const int iters = std::min(p.lanczos_iters, p.n);  // max_iter = min(max_iter, n), src/lanczos.cu:55
+    ProfScope ps(h, PSDP_PROF_LANCZOS);
+    HIPCHK(launch_lanczos(b.a32, 0, p.npad, p.n, p.npad, iters, tol, h->seed, margin, h->st, b.v, s));
+  } else {
+    HIPCHK(launch_set_scale(h->st, 1.0, s));
+  }
+  {
+    ProfScope ps(h, PSDP_PROF_PREP);
+    HIPCHK(launch_scale_split(b.a32, p.N, &h->st->inv_scale, sch.fmt[0], b.ap, sch.final_fmt, b.x32, b.xp, s));
+  }
+
+  for (int i = 0; i < sch.nsteps; ++i) {
+    const int f = sch.fmt[i];
+    const int fn = i + 1 < sch.nsteps ? sch.fmt[i + 1] : sch.final_fmt;
+    const bool last = i + 1 == sch.nsteps;
+    // A2 = A A
+    ProdDesc d1 = base_desc(p);
+    set_operand(d1.x, f, b.a32, b.ap);
+    set_operand(d1.y, f, b.a32, b.ap);
+    d1.mode = EPI_STORE;
+    d1.c32 = f == FMT_F32 ? b.p2 : nullptr;
+    d1.out_fmt = f;
+    for (int q = 0; q < 3; ++q) d1.outp[q] = b.p2p[q];
+    {
+      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
+      HIPCHK(launch_sym_product(f, p.sub, d1, tl.dev, tl.count, s));
+    }
+    // A3 = A2 A
+    ProdDesc d2 = base_desc(p);
+    set_operand(d2.x, f, b.p2, b.p2p);
+    set_operand(d2.y, f, b.a32, b.ap);
+    d2.mode = EPI_STORE;
+    d2.c32 = b.p3;
+    d2.out_fmt = f;
+    for (int q = 0; q < 3; ++q) d2.outp[q] = b.p3p[q];
+    {
+      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
+      HIPCHK(launch_sym_product(f, p.sub, d2, tl.dev, tl.count, s));
+    }
+    // A = a A + b A3 + c A3 A2 (last step: S = (I + A)/2)
+    ProdDesc d3 = base_desc(p);
+    set_operand(d3.x, f, b.p3, b.p3p);
+    set_operand(d3.y, f, b.p2, b.p2p);
+    d3.mode = last ? EPI_POLY_HALF_ID : EPI_POLY;
+    d3.a = sch.a[i];
+    d3.b = sch.b[i];
+    d3.c = sch.c[i];
+    d3.c32 = b.a32;
+    d3.p3 = b.p3;
+    d3.out_fmt = fn;
+    for (int q = 0; q < 3; ++q) d3.outp[q] = b.ap[q];
+    {
+      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
+      HIPCHK(launch_sym_product(f, p.sub, d3, tl.dev, tl.count, s));
+    }
+  }
+  // W = X_s S, Y = s W
+  ProdDesc d4 = base_desc(p);
+  set_operand(d4.x, sch.final_fmt, b.x32, b.xp);
+  set_operand(d4.y, sch.final_fmt, b.a32, b.ap);
+  d4.mode = EPI_OUTPUT;
+  d4.out = Y;
+  d4.out_ld = p.n;
+  d4.out_f64 = y_f64;
+  d4.scale = &h->st->scale;
+  {
+    ProfScope ps(h, PSDP_PROF_PROD_F32 + sch.final_fmt);
+    HIPCHK(launch_sym_product(sch.final_fmt, p.sub, d4, tl.dev, tl.count, s));
+  }
+  return PSDP_OK;
+}
+
+static void clear_profile(psdp_handle_t h) {
+  for (auto& r : h->prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  h->prof.clear();
+}
+
+static psdp_status_t check_flags(psdp_handle_t h) {
+  LanczosState st;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
+  if (st.nonfinite) return fail(PSDP_ERR_NONFINITE, "non-finite value in the input or the spectral bound");
+  return PSDP_OK;
+}
+
+static psdp_status_t project(psdp_handle_t h, const void* X, int x_f64, void* Y, int y_f64, int64_t n,
+                             const psdp_config_t* cfg, void* ws, size_t ws_bytes) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (!X || !Y) return fail(PSDP_ERR_INVALID_ARG, "null matrix pointer");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  Plan p;
+  STCHK(make_plan(cfg, n, p));
+  void* base = ws;
+  if (ws) {
+    if (ws_bytes < p.total)
+      return fail(PSDP_ERR_INVALID_ARG, "workspace of %zu bytes is smaller than the %zu bytes needed", ws_bytes,
+                  p.total);
+  } else {
+    STCHK(arena_reserve(h, p.total, &base));
+  }
+  Bufs b = carve(p, (char*)base);
+  STCHK(enqueue_projection(h, p, b, X, x_f64, Y, y_f64, cfg));
+  if (cfg->synchronize) STCHK(check_flags(h));
+  return PSDP_OK;
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* psdp_last_error(void) { return g_last_error.c_str(); }
+
+const char* psdp_status_string(psdp_status_t s) {
+  switch (s) {
+    case PSDP_OK: return "PSDP_OK";
+    case PSDP_ERR_INVALID_ARG: return "PSDP_ERR_INVALID_ARG";
+    case PSDP_ERR_NOMEM: return "PSDP_ERR_NOMEM";
+    case PSDP_ERR_NONFINITE: return "PSDP_ERR_NONFINITE";
+    case PSDP_ERR_UNSUPPORTED: return "PSDP_ERR_UNSUPPORTED";
+    case PSDP_ERR_DEVICE: return "PSDP_ERR_DEVICE";
+  }
+  return "PSDP_ERR_UNKNOWN";
+}
+
+psdp_status_t psdp_default_config(psdp_config_t* cfg, int method) {
+  if (!cfg) return fail(PSDP_ERR_INVALID_ARG, "null config");
+  memset(cfg, 0, sizeof *cfg);
+  cfg->method = method;
+  cfg->table = PSDP_TABLE_FP32;
+  for (int i = 0; i < PSDP_MAX_STEPS; ++i) cfg->step_format[i] = PSDP_FMT_F32;
+  cfg->final_format = PSDP_FMT_F32;
+  cfg->auto_scale = 1;
+  cfg->lanczos_iters = 20;
+  cfg->lanczos_tol = 1e-10;
+  cfg->scale_margin = 1.0;
+  cfg->synchronize = 1;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_create(psdp_handle_t* out, int device) {
+  if (!out) return fail(PSDP_ERR_INVALID_ARG, "null handle pointer");
+  *out = nullptr;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(PSDP_ERR_INVALID_ARG, "device %d not present", device);
+  HIPCHK(hipSetDevice(device));
+  psdp_handle_t h = new psdp_handle_st();
+  h->device = device;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(PSDP_ERR_DEVICE, "stream creation failed");
+  }
+  h->own_stream = true;
+  if (hipMalloc(&h->st, sizeof(LanczosState)) != hipSuccess) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return fail(PSDP_ERR_NOMEM, "state allocation failed");
+  }
+  (void)hipMemset(h->st, 0, sizeof(LanczosState));
+  *out = h;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_profile_enable(psdp_handle_t h, int on) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (on) clear_profile(h);
+  h->profiling = on != 0;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_profile_summary(psdp_handle_t h, int kind, int* launches, double* total_ms) {
+  if (!h || !launches || !total_ms) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  int c = 0;
+  double t = 0.0;
+  for (auto& r : h->prof) {
+    if (r.kind != kind) continue;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    ++c;
+    t += ms;
+  }
+  *launches = c;
+  *total_ms = t;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_destroy(psdp_handle_t h) {
+  if (!h) return PSDP_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  clear_profile(h);
+  for (auto& kv : h->tiles) (void)hipFree(kv.second.dev);
+  if (h->arena) (void)hipFree(h->arena);
+  if (h->st) (void)hipFree(h->st);
+  if (h->own_stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_set_stream(psdp_handle_t h, void* stream) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  if (h->own_stream) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamDestroy(h->stream);
+    h->own_stream = false;
+  }
+  h->stream = (hipStream_t)stream;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_get_stream(psdp_handle_t h, void** stream) {
+  if (!h || !stream) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  *stream = (void*)h->stream;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_set_seed(psdp_handle_t h, uint64_t seed) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  h->seed = seed;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_synchronize(psdp_handle_t h) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_workspace_size(const psdp_config_t* cfg, int64_t n, size_t* bytes) {
+  if (!bytes) return fail(PSDP_ERR_INVALID_ARG, "null output");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  Plan p;
+  STCHK(make_plan(cfg, n, p));
+  *bytes = p.total;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_last_bound(psdp_handle_t h, double* lo, double* up, int* iterations) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  LanczosState st;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
+  if (lo) *lo = st.lo;
+  if (up) *up = st.up;
+  if (iterations) *iterations = st.nb;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int64_t n, const psdp_config_t* cfg,
+                               void* ws, size_t ws_bytes) {
+  return project(h, X, 1, Y, 1, n, cfg, ws, ws_bytes);
+}
+
+psdp_status_t psdp_project_f32(psdp_handle_t h, const float* X, float* Y, int64_t n, const psdp_config_t* cfg,
+                               void* ws, size_t ws_bytes) {
+  return project(h, X, 0, Y, 0, n, cfg, ws, ws_bytes);
+}
+
+static psdp_status_t dropin(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes, int method,
+                            int auto_scale) {
+  psdp_config_t cfg;
+  psdp_default_config(&cfg, method);
+  cfg.auto_scale = auto_scale;
+  return project(h, mat, 1, mat, 1, n, &cfg, ws, ws_bytes);
+}
+
+psdp_status_t psdp_composite_fp32(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP32, 0);
+}
+psdp_status_t psdp_composite_fp32_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                             size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP32, 1);
+}
+psdp_status_t psdp_composite_fp16(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP16, 0);
+}
+psdp_status_t psdp_composite_fp16_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                             size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP16, 1);
+}
+psdp_status_t psdp_composite_fp32_emulated(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP32_EMULATED, 0);
+}
+psdp_status_t psdp_composite_fp32_emulated_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                                      size_t ws_bytes) {
+  return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP32_EMULATED, 1);
+}
+
+psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_t n, double* lo, double* up,
+                                        int64_t max_iter, double tol) {
+  if (!h || !A || !lo || !up) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0) return fail(PSDP_ERR_INVALID_ARG, "matrix size must be positive");
+  if (max_iter <= 0) max_iter = 20;
+  if (max_iter > LANCZOS_MAX_ITER)
+    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  const int iters = (int)std::min<int64_t>(max_iter, n);
+  const size_t vb = align256((size_t)n * 8);
+  void* base = nullptr;
+  STCHK(arena_reserve(h, vb * 6 + align256((size_t)n * 8 * iters), &base));
+  char* v = (char*)base;
+  LanczosVecs lv;
+  lv.q = (double*)(v + 0 * vb);
+  lv.w = (double*)(v + 1 * vb);
+  lv.w1 = (double*)(v + 2 * vb);
+  lv.vold = (double*)(v + 3 * vb);
+  lv.y = (double*)(v + 4 * vb);
+  lv.ay = (double*)(v + 5 * vb);
+  lv.V = (double*)(v + 6 * vb);
+  HIPCHK(launch_lanczos(A, 1, n, (int)n, (int)n, iters, tol, h->seed, 1.0, h->st, lv, h->stream));
+  LanczosState st;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
+  *lo = st.lo;
+  *up = st.up;
+  if (st.nonfinite) return fail(PSDP_ERR_NONFINITE, "non-finite value met by the Lanczos bound");
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float* Y, float* C, int64_t n,
+                                   int format) {
+  if (!h || !X || !Y || !C) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > (1 << 20)) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size");
+  if (format < 0 || format >= FMT_COUNT) return fail(PSDP_ERR_INVALID_ARG, "unknown format %d", format);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  const int sub = choose_sub(n), bt = 32 * sub;
+  const int npad = (int)((n + bt - 1) / bt * bt);
+  const int64_t N = (int64_t)npad * npad;
+  const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
+  void* base = nullptr;
+  STCHK(arena_reserve(h, 2 * f32b + 6 * u16b, &base));
+  char* c = (char*)base;
+  float* mx = (float*)c;
+  float* my = (float*)(c + f32b);
+  uint16_t* px[3];
+  uint16_t* py[3];
+  for (int i = 0; i < 3; ++i) {
+    px[i] = (uint16_t*)(c + 2 * f32b + i * u16b);
+    py[i] = (uint16_t*)(c + 2 * f32b + (3 + i) * u16b);
+  }
+  hipStream_t s = h->stream;
+  TileList tl;
+  STCHK(get_tiles(h, npad / bt, bt, tl));
+  HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, npad, npad, s));
+  HIPCHK(launch_sym_pad(Y, 0, n, (int)n, my, npad, npad, s));
+  uint16_t* none[3] = {nullptr, nullptr, nullptr};
+  HIPCHK(launch_scale_split(mx, N, nullptr, format, px, FMT_F32, mx, none, s));
+  HIPCHK(launch_scale_split(my, N, nullptr, format, py, FMT_F32, my, none, s));
+  ProdDesc d;
+  memset(&d, 0, sizeof d);
+  d.ld = npad;
+  d.npad = npad;
+  d.n = (int)n;
+  set_operand(d.x, format, mx, px);
+  set_operand(d.y, format, my, py);
+  d.mode = EPI_OUTPUT;
+  d.out = C;
+  d.out_ld = n;
+  d.out_f64 = 0;
+  d.scale = nullptr;
+  HIPCHK(launch_sym_product(format, sub, d, tl.dev, tl.count, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
+                                       const psdp_config_t* cfg) {
+  (void)h;
+  (void)blocks;
+  (void)count;
+  (void)cfg;
+  return fail(PSDP_ERR_UNSUPPORTED, "batched projection not built yet");
+}
+
+psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
+  (void)h;
+  (void)mat;
+  (void)n;
+  (void)eigenvalues;
+  return fail(PSDP_ERR_UNSUPPORTED, "eig_fp64_psd not built yet");
+}
+
+psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
+  (void)h;
+  (void)mat;
+  (void)n;
+  (void)eigenvalues;
+  return fail(PSDP_ERR_UNSUPPORTED, "eig_fp32_psd not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Device-side definitions shared by the HIP kernels of libpsdp (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "psdp_internal.h"
+
+namespace psdp {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Scalar conversions (round to nearest even).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint16_t b) {
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
+  return __uint_as_float(((uint32_t)b) << 16);
+}
+
+// Splits a float32 value into the pieces of operand format `fmt`.
+// F16X3: hi = f16(v), lo = f16((v - hi) * 2^11).  BF16X3: hi, v - hi.
+// BF16X9: hi, mid, lo (three bfloat16 terms).
+__device__ __forceinline__ void split_value(int fmt, float v, uint16_t out[3]) {
+  switch (fmt) {
+    case FMT_F16: out[0] = f32_to_f16_bits(v); break;
+    case FMT_F16X3: {
+      uint16_t h = f32_to_f16_bits(v);
+      out[0] = h;
+      out[1] = f32_to_f16_bits((v - f16_bits_to_f32(h)) * F16X3_LO_SCALE);
+      break;
+    }
+    case FMT_BF16: out[0] = f32_to_bf16_bits(v); break;
+    case FMT_BF16X3: {
+      uint16_t h = f32_to_bf16_bits(v);
+      out[0] = h;
+      out[1] = f32_to_bf16_bits(v - bf16_bits_to_f32(h));
+      break;
+    }
+    case FMT_BF16X9: {
+      uint16_t h = f32_to_bf16_bits(v);
+      float r = v - bf16_bits_to_f32(h);
+      uint16_t m = f32_to_bf16_bits(r);
+      out[0] = h;
+      out[1] = m;
+      out[2] = f32_to_bf16_bits(r - bf16_bits_to_f32(m));
+      break;
+    }
+    default: break;
+  }
+}
+
+__device__ __forceinline__ int fmt_pieces(int fmt) {
+  return fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : 1);
+}
+
+// Stores value v of element idx in operand format fmt.  For FMT_F32 the
+// value goes to f32 (piece pointers unused).
+__device__ __forceinline__ void store_operand(int fmt, float v, int64_t idx, float* f32,
+                                              uint16_t* const* pieces) {
+  if (fmt == FMT_F32) {
+    f32[idx] = v;
+    return;
+  }
+  uint16_t p[3];
+  split_value(fmt, v, p);
+  const int np = fmt_pieces(fmt);
+  pieces[0][idx] = p[0];
+  if (np > 1) pieces[1][idx] = p[1];
+  if (np > 2) pieces[2][idx] = p[2];
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block reductions (wave64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the whole block; every thread receives the result.  `red` must
+// hold blockDim.x/64 doubles of shared memory.  Deterministic order.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < nw; ++i) t += red[i];
+  return t;
+}
+
+}  // namespace psdp

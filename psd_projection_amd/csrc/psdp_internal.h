@@ -1,0 +1,87 @@
+// Types shared by the host driver and the HIP kernels of libpsdp.
+#pragma once
+#include <stdint.h>
+
+namespace psdp {
+
+// Operand formats of the symmetric products (values match psdp_format_t).
+enum : int {
+  FMT_F32 = 0,     // float32 operands on the FP32 MFMA
+  FMT_F16 = 1,     // binary16 operands, float32 accumulation
+  FMT_F16X3 = 2,   // hi + 2^-11 lo binary16 split, 3 products
+  FMT_BF16 = 3,    // bfloat16 operands
+  FMT_BF16X3 = 4,  // hi + lo bfloat16 split, 3 products
+  FMT_BF16X9 = 5,  // hi + mid + lo bfloat16 split, 9 products
+  FMT_COUNT = 6
+};
+
+constexpr float F16X3_LO_SCALE = 2048.0f;
+constexpr float F16X3_LO_INV = 1.0f / 2048.0f;
+
+// Epilogues of the symmetric product kernel.
+enum : int {
+  EPI_STORE = 0,       // C = X Y   -> c32 (optional) + operand pieces (out_fmt)
+  EPI_POLY = 1,        // A = a A + b P3 + c X Y (in place) + pieces of A
+  EPI_POLY_HALF_ID = 2,// as EPI_POLY, then S = (I + A)/2 written as pieces / c32
+  EPI_OUTPUT = 3       // out = scale * X Y on the logical n x n part (f64 or f32)
+};
+
+// One symmetric n_pad x n_pad product C = X Y of two symmetric matrices held
+// as operand pieces (row-major view, leading dimension ld, zero padded).
+struct ProdDesc {
+  const void* x[3];  // pieces of X (uint16 or float)
+  const void* y[3];  // pieces of Y
+  int64_t ld;        // leading dimension of every padded buffer
+  int npad;          // padded size (multiple of the tile)
+  int n;             // logical size
+  int mode;          // EPI_*
+  int out_fmt;       // operand format written by EPI_STORE / EPI_POLY*
+  float a, b, c;     // polynomial coefficients (EPI_POLY*)
+  float* c32;        // float32 result (EPI_STORE) or A in/out (EPI_POLY*)
+  const float* p3;   // float32 A^3 read by EPI_POLY*
+  uint16_t* outp[3]; // pieces written by EPI_STORE / EPI_POLY*
+  void* out;         // EPI_OUTPUT destination (column-major n x n, ld = out_ld)
+  int64_t out_ld;
+  int out_f64;       // 1: double output, 0: float output
+  const double* scale;  // EPI_OUTPUT multiplier (device scalar); null -> 1
+};
+
+// Tile of the upper triangle: matrix index + block coordinates (bi <= bj).
+struct TileRef {
+  int mat;
+  int bi;
+  int bj;
+  int pad;
+};
+
+constexpr int LANCZOS_MAX_ITER = 64;
+
+// Device-resident Lanczos state (one per matrix).
+struct LanczosState {
+  double alpha[LANCZOS_MAX_ITER];
+  double beta[LANCZOS_MAX_ITER];
+  double u[LANCZOS_MAX_ITER];  // eigenvector of the largest Ritz value
+  double beta_old;
+  double theta;
+  double lo, up;
+  double scale;      // s = up > 0 ? up : 1 (times the caller's margin)
+  double inv_scale;  // 1 / s
+  double rnorm;
+  int nb;            // completed iterations (nb_iter)
+  int done;          // loop finished (break rule hit)
+  int skip_ritz;     // nb == 0: no Ritz pair (all-zero input)
+  int nonfinite;     // non-finite value met
+};
+
+// Device vectors of one Lanczos run (length m each; V is m x max_iter).
+struct LanczosVecs {
+  double* q;
+  double* w;     // A q, then A A q
+  double* w1;
+  double* vold;
+  double* V;     // column j at V + j*m
+  double* y;
+  double* ay;
+};
+
+}  // namespace psdp

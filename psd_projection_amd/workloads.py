@@ -1,0 +1,66 @@
+"""Synthetic workloads of the benchmark configurations, generated on the GPU.
+
+Every matrix is X = Q diag(lam) Q^T with Q Haar-distributed (QR of a seeded
+Gaussian, sign-fixed so that diag(R) > 0), symmetrised; the exact projection
+Pi(X) = Q max(lam, 0) Q^T comes with it.  torch is used as plumbing (FP64
+QR/GEMM on the device); the projection itself runs in libpsdp.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def spectrum(n: int, seed: int, kind: str) -> np.ndarray:
+    """Eigenvalues of the configurations:
+    uniform: U[-1,1] (configs 1, 4); cfg2: N(0,1) scaled to max|lam| = 10 with
+    30% replaced by U[-1e-2,1e-2]; cfg3: U[-5,5] with 10% replaced by
+    U[-1e-3,1e-3]; normal: N(0,1) (config 5)."""
+    rng = np.random.default_rng(seed + 100)
+    if kind == "uniform":
+        return rng.uniform(-1.0, 1.0, n)
+    if kind == "normal":
+        return rng.standard_normal(n)
+    if kind == "cfg2":
+        lam = rng.standard_normal(n)
+        lam *= 10.0 / np.max(np.abs(lam))
+        idx = rng.choice(n, int(round(0.3 * n)), replace=False)
+        lam[idx] = rng.uniform(-1e-2, 1e-2, idx.size)
+        return lam
+    if kind == "cfg3":
+        lam = rng.uniform(-5.0, 5.0, n)
+        idx = rng.choice(n, int(round(0.1 * n)), replace=False)
+        lam[idx] = rng.uniform(-1e-3, 1e-3, idx.size)
+        return lam
+    raise ValueError(kind)
+
+
+def haar(n: int, seed: int, device="cuda"):
+    import torch
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    G = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+    Q, R = torch.linalg.qr(G)
+    del G
+    Q.mul_(torch.sign(torch.diagonal(R)).unsqueeze(0))
+    del R
+    return Q
+
+
+def spectral_matrix(n: int, seed: int, kind: str, device="cuda", want_projection=True):
+    """Returns (X float64, Pi(X) float64 or None, lam numpy)."""
+    import torch
+    lam = spectrum(n, seed, kind)
+    Q = haar(n, seed, device)
+    L = torch.from_numpy(lam).to(device)
+    X = (Q * L) @ Q.T
+    X = (X + X.T) * 0.5
+    Pm = None
+    if want_projection:
+        Pm = (Q * L.clamp(min=0)) @ Q.T
+    del Q
+    return X, Pm, lam
+
+
+def rel_fro_error(Y, Pm) -> float:
+    import torch
+    d = (Y.to(torch.float64) - Pm).norm()
+    return float(d / Pm.norm())

@@ -1,0 +1,264 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Tolerances:
+  * single products: relative Frobenius error vs the oracle's rounding-emulated
+    product (same operand rounding, float32 accumulation) <= 2e-6;
+  * projections: the reference's own test bars (rel <= 1e-3 / abs <= 0.1,
+    test/composite_FP32_test.hpp) AND the oracle's own error for the same
+    method and input within a stated margin (the GPU must reproduce the
+    method's accuracy, not just pass 1e-3);
+  * Lanczos (FP64 input, same start vector): lo/up to 1e-9 relative.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import psdp_oracle as O
+
+torch = pytest.importorskip("torch")
+import psd_projection_amd as P  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+KATS = json.loads((GOLD / "reference_kats.json").read_text())
+DEV = "cuda:0"
+
+
+def to_dev(M, dtype=torch.float64):
+    # column-major n x n == transpose of the row-major array; matrices here
+    # are symmetric, but keep the layout explicit for non-symmetric inputs
+    return torch.from_numpy(np.asfortranarray(M).T.copy()).to(DEV, dtype)
+
+
+def from_dev(T):
+    return T.detach().cpu().numpy().astype(np.float64).T.copy()
+
+
+def splitmix64(z):
+    m = (1 << 64) - 1
+    z = (z + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def start_vector(seed, n):
+    """The library's Lanczos start vector: q_i ~ U(0,1] from a splitmix64
+    counter stream (lanczos.hip, lanczos_init_kernel)."""
+    m = (1 << 64) - 1
+    return np.array([((splitmix64(seed ^ ((0x632BE59BD9B4E019 * (i + 1)) & m)) >> 11) + 1) * 2.0 ** -53
+                     for i in range(n)])
+
+
+# ------------------------------------------------------------------ products
+@pytest.mark.parametrize("fmt", ["f32", "f16", "f16x3", "bf16", "bf16x3", "bf16x9"])
+@pytest.mark.parametrize("n", [64, 200, 1100])
+def test_sym_product(handle, fmt, n):
+    print(f"sym_product fmt={fmt} n={n}", flush=True)
+    X, _, _ = O.spectral_input(n, 100 + n, "uniform")
+    X32 = X.astype(np.float32)
+    Y32 = (X @ X).astype(np.float32)
+    Y32 = ((Y32 + Y32.T) / 2).astype(np.float32)
+    C = from_dev(P.sym_product(handle, to_dev(X32, torch.float32), to_dev(Y32, torch.float32), fmt))
+    ref = O.emulated_product(X32, Y32, fmt).astype(np.float64)
+    assert np.array_equal(C, C.T)
+    err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
+    assert err <= 2e-6, err
+
+
+# ------------------------------------------------------------------ KATs
+def test_kat_deterministic_3x3(handle):
+    k = KATS["composite_fp32_deterministic"]
+    X = np.array(k["input_colmajor"]).reshape(3, 3, order="F")
+    d = to_dev(X)
+    P.composite_FP32(handle, d)
+    E = np.array(k["expected_colmajor"]).reshape(3, 3, order="F")
+    assert np.abs(from_dev(d) - E).max() <= k["tol"]
+
+
+@pytest.mark.parametrize("fn,tol_scale", [(P.composite_FP32_auto_scale, 1.0),
+                                          (P.composite_FP16_auto_scale, 5.0),
+                                          (P.composite_FP32_emulated_auto_scale, 1.0)])
+def test_kat_simple5(handle, fn, tol_scale):
+    k = KATS["simple5"]
+    d = to_dev(np.diag(k["diagonal"]))
+    fn(handle, d)
+    Y = from_dev(d)
+    assert np.abs(Y - np.diag(k["expected_diagonal"])).max() <= k["tol"] * tol_scale
+
+
+# ------------------------------------------------------------------ Lanczos
+def test_lanczos_kat_5x5(handle):
+    k = KATS["lanczos_5x5"]
+    A = np.array(k["input_colmajor"]).reshape(5, 5, order="F")
+    seed = 99
+    handle.set_seed(seed)
+    lo, up = P.approximate_two_norm(handle, to_dev(A), max_iter=k["max_iter"], tol=k["tol"])
+    lo_o, up_o = O.lanczos_two_norm(A, k["max_iter"], k["tol"], q0=start_vector(seed, 5))
+    assert lo <= k["two_norm"] + 1e-9 <= up
+    assert lo == pytest.approx(lo_o, rel=1e-9)
+    assert up == pytest.approx(up_o, rel=1e-9)
+
+
+@pytest.mark.parametrize("n,kind,seed", [(300, "normal", 3), (1000, "cfg2", 4), (777, "cfg3", 5)])
+def test_lanczos_matches_oracle(handle, n, kind, seed):
+    X, Q, lam = O.spectral_input(n, seed, kind)
+    handle.set_seed(seed)
+    lo, up = P.approximate_two_norm(handle, to_dev(X))
+    lo_o, up_o = O.lanczos_two_norm(X, 20, 1e-10, q0=start_vector(seed, n))
+    assert lo == pytest.approx(lo_o, rel=1e-9)
+    assert up == pytest.approx(up_o, rel=1e-9)
+    assert lo <= np.abs(lam).max() * (1 + 1e-12) <= up
+
+
+def test_lanczos_zero_matrix(handle):
+    lo, up = P.approximate_two_norm(handle, torch.zeros(50, 50, dtype=torch.float64, device=DEV))
+    assert (lo, up) == (0.0, 1.0)
+
+
+# ------------------------------------------------------------------ projections
+def run_projection(handle, X, method, dtype=torch.float64, **kw):
+    Xd = to_dev(X, dtype)
+    Y = P.project(handle, Xd, method=method, **kw)
+    return from_dev(Y)
+
+
+def oracle_error(X, P_true, method, seed):
+    n = X.shape[0]
+    _, up = O.lanczos_two_norm(X, 20, 1e-10, q0=start_vector(seed, n))
+    Xs = X / up
+    if method == "composite_FP32":
+        Y = O.composite_FP32(Xs)
+    elif method == "composite_FP16":
+        Y = O.composite_FP16(Xs)
+    elif method == "composite_FP32_emulated":
+        Y = O.composite_FP32_emulated(Xs)
+    elif method == "mixed":
+        c = O.fp16_chain_coefficients()
+        Y = O.composite_chain(Xs, c, ["f16x3"] * 3 + ["f16"] * 4, "f16")
+    return O.rel_fro_error(Y * up, P_true)
+
+
+def test_reference_uniform_scaled_1000(handle):
+    # test/composite_FP32_test.hpp:80-129: n=1000, radius 1, no scaling
+    A, Ppsd, _, _ = O.generate_and_project(1000, 1.0)
+    d = to_dev(A)
+    P.composite_FP32(handle, d)
+    Y = from_dev(d)
+    err = O.rel_fro_error(Y, Ppsd)
+    e_or = O.rel_fro_error(O.composite_FP32(A), Ppsd)
+    print("uniform_scaled rel err gpu", err, "oracle", e_or)
+    assert err <= 1e-3
+    assert err <= 1.5 * e_or + 2e-6
+
+
+@pytest.mark.parametrize("fn,method", [(P.composite_FP32_auto_scale, "composite_FP32"),
+                                       (P.composite_FP32_emulated_auto_scale, "composite_FP32_emulated")])
+def test_reference_uniform_autoscaled_1024(handle, fn, method):
+    # test/composite_FP32_test.hpp:201-247: n=1024, radius 10, abs err <= 0.1
+    A, Ppsd, _, _ = O.generate_and_project(1024, 10.0)
+    seed = 17
+    handle.set_seed(seed)
+    d = to_dev(A)
+    fn(handle, d)
+    Y = from_dev(d)
+    abs_err = np.linalg.norm(Y - Ppsd)
+    err = abs_err / np.linalg.norm(Ppsd)
+    e_or = oracle_error(A, Ppsd, method, seed)
+    print(method, "autoscaled1024 abs", abs_err, "rel", err, "oracle rel", e_or)
+    assert abs_err <= 0.1
+    assert err <= 1.5 * e_or + 2e-6
+
+
+@pytest.mark.parametrize("method,kind,n,bar", [
+    ("composite_FP32", "uniform", 512, 2e-5),
+    ("composite_FP32_emulated", "uniform", 512, 2e-5),
+    ("composite_FP16", "cfg2", 512, None),
+    ("mixed", "cfg2", 1024, 1e-3),
+    ("mixed", "cfg3", 1024, 1e-3),
+    ("mixed", "uniform", 640, 1e-3),
+])
+def test_projection_accuracy(handle, method, kind, n, bar):
+    print(f"projection {method} {kind} n={n}", flush=True)
+    X, Q, lam = O.spectral_input(n, 1 if kind == "uniform" else (2 if kind == "cfg2" else 3), kind)
+    Ptrue = O.closed_form_projection(Q, lam)
+    seed = 5
+    handle.set_seed(seed)
+    Y = run_projection(handle, X, method)
+    err = O.rel_fro_error(Y, Ptrue)
+    e_or = oracle_error(X, Ptrue, method, seed)
+    print(f"  gpu rel err {err:.3e} oracle {e_or:.3e}")
+    assert np.array_equal(Y, Y.T)
+    if bar is not None:
+        assert err <= bar
+    # same method, same rounding: within 30% (+ float32 accumulation noise)
+    assert err <= 1.3 * e_or + 2e-6
+    assert err >= 0.7 * e_or - 2e-6
+
+
+def test_config1_fp32_io(handle):
+    # config 1: n=512, FP32 storage, composite_FP32 with the Lanczos bound
+    X, Q, lam = O.config1_input(512)
+    Ptrue = O.closed_form_projection(Q, lam)
+    Y = run_projection(handle, X, "composite_FP32", dtype=torch.float32)
+    err = O.rel_fro_error(Y, Ptrue)
+    print("config1 fp32 rel err", err)
+    assert err <= 2e-5
+
+
+# ------------------------------------------------------------------ edges
+def test_zero_matrix(handle):
+    Z = torch.zeros(70, 70, dtype=torch.float64, device=DEV)
+    for m in ("composite_FP32", "composite_FP16", "mixed", "composite_FP32_emulated"):
+        Y = P.project(handle, Z, method=m)
+        assert torch.count_nonzero(Y).item() == 0
+
+
+@pytest.mark.parametrize("val", [-3.0, 2.0, 0.25])
+def test_one_by_one(handle, val):
+    d = torch.tensor([[val]], dtype=torch.float64, device=DEV)
+    P.composite_FP32_auto_scale(handle, d)
+    assert d.item() == pytest.approx(max(val, 0.0), abs=1e-4 * abs(val))
+
+
+def test_two_by_two(handle):
+    X = np.array([[1.0, 2.0], [2.0, -1.0]])
+    w, V = np.linalg.eigh(X)
+    expect = (V * np.maximum(w, 0)) @ V.T
+    Y = run_projection(handle, X, "composite_FP32")
+    assert np.abs(Y - expect).max() < 1e-4
+
+
+def test_nonfinite_input_reported(handle):
+    X = torch.eye(40, dtype=torch.float64, device=DEV)
+    X[3, 5] = float("nan")
+    with pytest.raises(P.PsdpError) as e:
+        P.composite_FP32_auto_scale(handle, X)
+    assert e.value.status == P.ERR_NONFINITE
+
+
+def test_in_place_equals_out_of_place_and_deterministic(handle):
+    X, _, _ = O.spectral_input(300, 9, "normal")
+    handle.set_seed(3)
+    a = P.project(handle, to_dev(X), method="mixed")
+    b = to_dev(X)
+    handle.set_seed(3)
+    P.project(handle, b, out=b, method="mixed")
+    handle.set_seed(3)
+    c = P.project(handle, to_dev(X), method="mixed")
+    assert torch.equal(a, b) and torch.equal(a, c)
+
+
+def test_explicit_workspace(handle):
+    X, Q, lam = O.spectral_input(256, 4, "uniform")
+    need = P.workspace_size(256, "composite_FP32")
+    ws = torch.empty(need, dtype=torch.uint8, device=DEV)
+    handle.set_seed(1)
+    Y1 = P.project(handle, to_dev(X), method="composite_FP32", workspace=ws)
+    handle.set_seed(1)
+    Y2 = P.project(handle, to_dev(X), method="composite_FP32")
+    assert torch.equal(Y1, Y2)
+    with pytest.raises(P.PsdpError):
+        P.project(handle, to_dev(X), method="composite_FP32", workspace=ws[: need // 2])

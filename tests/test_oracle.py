@@ -1,0 +1,259 @@
+"""Pins the CPU oracle (oracle/psdp_oracle.py) before any GPU result is
+compared with it: reference known-answer tests, values fixed by the
+mathematics, independent library routines, and planted mistakes that must be
+caught.  CPU only."""
+import ctypes
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import psdp_oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+KATS = json.loads((GOLD / "reference_kats.json").read_text())
+
+
+# ----------------------------------------------------------------- generators
+def test_glibc_rand_matches_libc():
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    expect = [libc.rand() for _ in range(5000)]
+    got = O.GlibcRandom(1).take(5000).tolist()
+    assert got == expect
+    # the well-known first outputs after srand(1)
+    assert got[:3] == [1804289383, 846930886, 1681692777]
+
+
+def test_glibc_rand_chunked_matches_single_call():
+    g = O.GlibcRandom(1)
+    a = np.concatenate([g.take(7), g.take(100), g.take(1)])
+    assert a.tolist() == O.GlibcRandom(1).take(108).tolist()
+
+
+def test_generate_and_project_spectrum():
+    A, P, Q, lam = O.generate_and_project(40, radius=10.0)
+    assert np.allclose(A, A.T, atol=0)
+    assert np.allclose(np.linalg.eigvalsh(A), np.sort(lam), atol=1e-12)
+    assert np.allclose(Q.T @ Q, np.eye(40), atol=1e-13)
+    assert np.allclose(P, O.closed_form_projection(Q, lam), atol=1e-12)
+
+
+# ----------------------------------------------------------------- rounding
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32),
+                        np.array([1 + 2**-8, 1 + 3 * 2**-8, 1 + 2**-9, -1 - 2**-8, 0.0, 1e-40], np.float32)])
+    t = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(O.round_bf16(x), t)
+
+
+def test_f16_rounding_ties_to_even():
+    x = np.array([1 + 2**-11, 1 + 3 * 2**-11, 2**-25, 3 * 2**-26], np.float32)
+    assert O.round_f16(x).tolist() == [1.0, 1 + 2**-9, 0.0, 2**-24]
+
+
+@pytest.mark.parametrize("fmt,bits", [("f16x3", 21), ("bf16x3", 15), ("bf16x9", 23)])
+def test_split_reconstructs(fmt, bits):
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal(10000) * 0.01).astype(np.float32)
+    p = O.split_operand(x, fmt)
+    if fmt == "f16x3":
+        r = p[0].astype(np.float64) + p[1].astype(np.float64) / O.F16X3_LO_SCALE
+    else:
+        r = sum(q.astype(np.float64) for q in p)
+    # absolute error relative to the magnitude of the operand (tiny entries
+    # fall into the binary16 subnormal range, where relative precision drops)
+    assert np.max(np.abs(r - x)) / np.abs(x).max() < 2.0 ** -bits
+
+
+def test_emulated_products():
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((96, 96)).astype(np.float32) / 10
+    Y = rng.standard_normal((96, 96)).astype(np.float32) / 10
+    exact = X.astype(np.float64) @ Y.astype(np.float64)
+    err = lambda f: np.abs(O.emulated_product(X, Y, f) - exact).max() / np.abs(exact).max()
+    # float32 accumulation bounds everything at ~1e-7..1e-6; the rounding of
+    # the operands sets the rest (2^-11 f16, 2^-8 bf16, 2^-16 bf16x3).
+    assert err("f32") < 1e-6
+    assert err("bf16x9") < 1e-6
+    assert err("f16x3") < 1e-6
+    assert 1e-6 < err("bf16x3") < 2e-5
+    assert 1e-4 < err("f16") < 1e-3
+    assert 1e-3 < err("bf16") < 1e-2
+
+
+# ----------------------------------------------------------------- chain
+def test_transfer_function_max_error():
+    # max |g - ReLU| on [-1, 1]: 4.48e-6 (FP32 table), 4.65e-5 (FP16 table)
+    x = np.linspace(-1, 1, 200001)
+    relu = np.maximum(x, 0)
+    e32 = np.abs(O.transfer_function(x, O.fp32_chain_coefficients()) - relu).max()
+    e16 = np.abs(O.transfer_function(x, O.fp16_chain_coefficients()) - relu).max()
+    assert e32 == pytest.approx(4.48e-6, rel=0.01)
+    assert e16 == pytest.approx(4.65e-5, rel=0.01)
+
+
+def test_transfer_function_shape():
+    c = O.fp32_chain_coefficients()
+    P = lambda x: O.polynomial_chain_scalar(np.array(x), c)
+    xs = np.linspace(1e-3, 1.0, 1000)
+    assert np.abs(P(xs) - 1).max() < 1e-5
+    assert np.allclose(P(-xs), -P(xs), atol=0)  # odd
+    assert P(1e-4) == pytest.approx(1.0013, abs=2e-3)
+    assert P(3e-5) == pytest.approx(0.775, abs=5e-3)
+    assert P(1e-5) == pytest.approx(0.31, abs=1e-2)
+
+
+@pytest.mark.parametrize("table,ref", [(O.COEFF_FP32_TABLE, 4.48e-6), (O.COEFF_FP16_TABLE, 4.65e-5)])
+def test_coefficient_scaling_planted_mistake(table, ref):
+    # Planted mistake: coefficients without the 1.001 / 1.01 scale factor
+    # miss the max-error value above by more than its tolerance.
+    bad = O.step_coefficients(table, 1.0, 0)
+    x = np.linspace(-1, 1, 200001)
+    e_bad = np.abs(O.transfer_function(x, bad) - np.maximum(x, 0)).max()
+    assert abs(e_bad / ref - 1) > 0.02
+
+
+def test_chain_matches_closed_form():
+    d = np.load(GOLD / "chain_small.npz")
+    for n in (64, 128):
+        X, Q, lam = d[f"X{n}"], d[f"Q{n}"], d[f"lam{n}"]
+        for table, coeffs in (("fp32", O.fp32_chain_coefficients()), ("fp16", O.fp16_chain_coefficients())):
+            Y = O.composite_exact(X, table)
+            G = (Q * O.transfer_function(lam, coeffs)) @ Q.T
+            assert np.abs(Y - G).max() < 1e-12
+            assert np.abs(Y - d[f"chain_{table}_table_f64_{n}"]).max() < 1e-13
+
+
+def test_chain_planted_mistake_caught():
+    d = np.load(GOLD / "chain_small.npz")
+    X, Q, lam = d["X64"], d["Q64"], d["lam64"]
+    c = O.fp32_chain_coefficients()
+    wrong = [(a, b, 0.0) for a, b, _ in c]  # dropped c x^5 term
+    with np.errstate(all="ignore"):
+        Y = O.composite_chain(X, wrong, ["f64"] * len(c), "f64")
+    G = (Q * O.transfer_function(lam, c)) @ Q.T
+    assert not np.all(np.abs(Y - G) <= 1e-3)
+
+
+def test_kat_deterministic_3x3():
+    k = KATS["composite_fp32_deterministic"]
+    X = np.array(k["input_colmajor"]).reshape(3, 3, order="F")
+    Y = O.composite_FP32(X)
+    E = np.array(k["expected_colmajor"]).reshape(3, 3, order="F")
+    assert np.abs(Y - E).max() <= k["tol"]
+
+
+@pytest.mark.parametrize("method", ["fp32", "fp16", "fp32_emulated", "exact"])
+def test_kat_simple5(method):
+    # The reference checks this KAT at 1e-3 with an FP32 chain; binary16
+    # operands carry 2^-11 relative error on entries up to 5, so the FP16
+    # chain is held to 1e-3 of the largest entry instead.
+    k = KATS["simple5"]
+    X = np.diag(k["diagonal"])
+    Y = O.composite_auto_scale(X, method, q0=np.ones(5))
+    tol = k["tol"] * (5.0 if method == "fp16" else 1.0)
+    assert np.abs(Y - np.diag(k["expected_diagonal"])).max() <= tol
+
+
+def test_fp32_chain_error_level():
+    X, Q, lam = O.spectral_input(256, 1, "uniform")
+    P = O.closed_form_projection(Q, lam)
+    _, up = O.lanczos_two_norm(X)
+    err = O.rel_fro_error(O.composite_FP32(X / up) * up, P)
+    assert 1e-6 < err < 2e-5
+
+
+def test_fp16_chain_error_level_clustered():
+    # The reference FP16 chain misses 1e-3 on a cfg2-like clustered spectrum.
+    X, Q, lam = O.spectral_input(512, 2, "cfg2")
+    P = O.closed_form_projection(Q, lam)
+    _, up = O.lanczos_two_norm(X)
+    err = O.rel_fro_error(O.composite_FP16(X / up) * up, P)
+    assert 1.5e-3 < err < 5e-3
+
+
+# ----------------------------------------------------------------- Lanczos
+def test_lanczos_kat_5x5():
+    k = KATS["lanczos_5x5"]
+    A = np.array(k["input_colmajor"]).reshape(5, 5, order="F")
+    assert k["two_norm"] == pytest.approx(7.089012550562305, abs=1e-12)
+    lo, up = O.lanczos_two_norm(A, k["max_iter"], k["tol"], q0=np.ones(5))
+    assert lo <= k["two_norm"] + 1e-9 <= up + 1e-9
+    # n = 5: the fifth beta vanishes, the break rule stops at nb = 4, and the
+    # 4x4 Krylov space already pins the top Ritz value to ~1e-7.
+    assert lo == pytest.approx(k["two_norm"], abs=1e-6)
+
+
+@pytest.mark.parametrize("lam", [-10.0, 3.0, 0.5, -0.7])
+@pytest.mark.parametrize("iters", [2, 3, 4])
+def test_lanczos_matches_rayleigh_ritz(lam, iters):
+    # Independent route to the same bound: an orthonormal basis of the Krylov
+    # space span{q, B q, ..., B^(k-1) q} of B = A^T A by QR, the top
+    # Rayleigh-Ritz pair (theta, y) of B on it, and the reference's residual
+    # r = A y - theta y (src/lanczos.cu:255 copies ry into w1).
+    rng = np.random.default_rng(7)
+    d = np.concatenate([[lam], rng.uniform(-abs(lam) / 2, abs(lam) / 2, 39)])
+    Qr, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    A = (Qr * d) @ Qr.T
+    q0 = np.ones(40)
+    B = A.T @ A
+    K = np.stack([np.linalg.matrix_power(B, j) @ q0 for j in range(iters)], axis=1)
+    Qk, _ = np.linalg.qr(K)
+    w, U = np.linalg.eigh(Qk.T @ B @ Qk)
+    y = Qk @ U[:, -1]
+    up_rr = np.sqrt(w[-1] + np.linalg.norm(A @ y - w[-1] * y))
+    lo, up = O.lanczos_two_norm(A, iters, 0.0, q0=q0)
+    assert lo == pytest.approx(np.sqrt(w[-1]), rel=1e-9)
+    assert up == pytest.approx(up_rr, rel=1e-7)
+    # planted mistake: the textbook residual B y - theta y gives another bound
+    up_textbook = np.sqrt(w[-1] + np.linalg.norm(B @ y - w[-1] * y))
+    assert abs(up_textbook - up) > 1e-3 * up
+
+
+def test_lanczos_zero_matrix():
+    assert O.lanczos_two_norm(np.zeros((6, 6))) == (0.0, 1.0)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lanczos_brackets_norm(seed):
+    rng = np.random.default_rng(seed)
+    n = 200
+    G = rng.standard_normal((n, n))
+    A = (G + G.T) / 2
+    if seed % 2:
+        A -= 3 * np.abs(np.linalg.eigvalsh(A)).max() * np.outer(np.ones(n), np.ones(n)) / n
+    nrm = np.abs(np.linalg.eigvalsh(A)).max()
+    lo, up = O.lanczos_two_norm(A, 20, 1e-10)
+    assert lo <= nrm * (1 + 1e-12)
+    assert up >= nrm
+
+
+# ----------------------------------------------------------------- EVD
+def test_eig_fp64_kats():
+    k = KATS["simple5"]
+    Y, w = O.eig_FP64_psd(np.diag(k["diagonal"]), return_eigenvalues=True)
+    assert np.abs(Y - np.diag(k["expected_diagonal"])).max() < 1e-14
+    assert np.allclose(w, np.sort(k["diagonal"]))
+    k3 = KATS["composite_fp32_deterministic"]
+    X = np.array(k3["input_colmajor"]).reshape(3, 3, order="F")
+    assert np.abs(O.eig_FP64_psd(X) - X).max() < 1e-14
+
+
+def test_eig_fp64_closed_form():
+    X, Q, lam = O.spectral_input(150, 5, "normal")
+    assert np.abs(O.eig_FP64_psd(X) - O.closed_form_projection(Q, lam)).max() < 1e-12
+
+
+def test_eig_fp32_close():
+    X, Q, lam = O.spectral_input(120, 6, "normal")
+    assert O.rel_fro_error(O.eig_FP32_psd(X), O.closed_form_projection(Q, lam)) < 1e-5
+
+
+def test_kat_lobpcg_eigenvalues():
+    k = KATS["lobpcg_6x6"]
+    ev = sorted(k["eigenvalues"], reverse=True)[:2]
+    assert np.allclose(ev, k["expected_top2_reference"], atol=k["tol"])
