@@ -1,0 +1,111 @@
+// psd_projection.hpp -- C++ drop-ins with the reference's names and argument
+// order, header-only over the C ABI of psdp.h.
+//
+// Reference: ComputationalRobotics/psd_projection, include/psd_projection/*.h.
+// The cuBLAS and cuSOLVER handles of the reference become one psdp_handle_t
+// (the overloads below take it in both handle positions, so call sites keep
+// their shape); matrices are device pointers, column-major, in place.  As in
+// the reference (include/psd_projection/check.h:18-57) a failing call prints
+// the error with its location and returns; use the psdp_* C functions to get
+// the status instead.
+#ifndef PSD_PROJECTION_PSD_PROJECTION_HPP
+#define PSD_PROJECTION_PSD_PROJECTION_HPP
+
+#include <cstddef>
+#include <cstdio>
+
+#include "psd_projection/psdp.h"
+
+namespace psd_projection {
+
+namespace detail {
+inline void report(psdp_status_t s, const char* what) {
+  if (s != PSDP_OK) std::fprintf(stderr, "psdp error %s in %s: %s\n", psdp_status_string(s), what, psdp_last_error());
+}
+// The reference sizes workspaces as 3 n^2 floats; pass such a buffer on only
+// when it is large enough for this library's layout, else use the handle's
+// arena.
+inline size_t usable_ws(int method, int64_t n, const void* ws, size_t ws_bytes) {
+  if (!ws) return 0;
+  psdp_config_t cfg;
+  psdp_default_config(&cfg, method);
+  size_t need = 0;
+  if (psdp_workspace_size(&cfg, n, &need) != PSDP_OK) return 0;
+  return ws_bytes >= need ? ws_bytes : 0;
+}
+}  // namespace detail
+
+// composite_FP32.h:12-17
+inline void composite_FP32(psdp_handle_t h, double* mat, const int mat_size, float* workspace = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP32, mat_size, workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp32(h, mat, mat_size, b ? workspace : nullptr, b), "composite_FP32");
+}
+
+// composite_FP32.h:24-30
+inline void composite_FP32_auto_scale(psdp_handle_t h, psdp_handle_t /*solverH*/, double* mat, const int mat_size,
+                                      float* workspace = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP32, mat_size, workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp32_auto_scale(h, mat, mat_size, b ? workspace : nullptr, b),
+                 "composite_FP32_auto_scale");
+}
+
+// composite_FP16.h:12-18 (the half workspace of the reference is not needed)
+inline void composite_FP16(psdp_handle_t h, double* mat, const int n, float* float_workspace = nullptr,
+                           void* /*half_workspace*/ = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP16, n, float_workspace,
+                                     3 * sizeof(float) * (size_t)n * n);
+  detail::report(psdp_composite_fp16(h, mat, n, b ? float_workspace : nullptr, b), "composite_FP16");
+}
+
+// composite_FP16.h:26-33
+inline void composite_FP16_auto_scale(psdp_handle_t h, psdp_handle_t /*solverH*/, double* mat, const int mat_size,
+                                      float* float_workspace = nullptr, void* /*half_workspace*/ = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP16, mat_size, float_workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp16_auto_scale(h, mat, mat_size, b ? float_workspace : nullptr, b),
+                 "composite_FP16_auto_scale");
+}
+
+// composite_FP32_emulated.h:17-22 (no CUDA-version guard: always available)
+inline void composite_FP32_emulated(psdp_handle_t h, double* mat, const int mat_size, float* workspace = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP32_EMULATED, mat_size, workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp32_emulated(h, mat, mat_size, b ? workspace : nullptr, b),
+                 "composite_FP32_emulated");
+}
+
+// composite_FP32_emulated.h:29-35
+inline void composite_FP32_emulated_auto_scale(psdp_handle_t h, psdp_handle_t /*solverH*/, double* mat,
+                                               const int mat_size, float* workspace = nullptr) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP32_EMULATED, mat_size, workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp32_emulated_auto_scale(h, mat, mat_size, b ? workspace : nullptr, b),
+                 "composite_FP32_emulated_auto_scale");
+}
+
+// lanczos.h:11-17
+inline void approximate_two_norm(psdp_handle_t h, psdp_handle_t /*cusolverH*/, const double* A, size_t n,
+                                 double* lo, double* up, size_t max_iter = 20, double tol = 1e-10) {
+  detail::report(psdp_approximate_two_norm(h, A, (int64_t)n, lo, up, (int64_t)max_iter, tol),
+                 "approximate_two_norm");
+}
+
+// eig_FP64_psd.h:14-20.  With return_eigenvalues the reference returns a
+// device array the caller frees; here the caller passes that array (n
+// doubles) or nullptr, and nothing uninitialised is ever returned.
+inline void eig_FP64_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
+                         double* eigenvalues_out = nullptr) {
+  detail::report(psdp_eig_fp64_psd(h, mat, (int64_t)mat_size, eigenvalues_out), "eig_FP64_psd");
+}
+
+// eig_FP32_psd.h:14-20
+inline void eig_FP32_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
+                         double* eigenvalues_out = nullptr) {
+  detail::report(psdp_eig_fp32_psd(h, mat, (int64_t)mat_size, eigenvalues_out), "eig_FP32_psd");
+}
+
+}  // namespace psd_projection
+
+#endif  // PSD_PROJECTION_PSD_PROJECTION_HPP

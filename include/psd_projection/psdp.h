@@ -7,7 +7,7 @@
  * enqueued on the handle's HIP stream.
  *
  * Drop-in map (reference = ComputationalRobotics/psd_projection,
- * include/psd_projection/*.h; cuBLAS/cuSOLVER handles become one
+ * the headers in include/psd_projection; cuBLAS/cuSOLVER handles become one
  * psdp_handle_t; `float* workspace` becomes (ws, ws_bytes)):
  *
  *   psdp_composite_fp32                  <- composite_FP32              composite_FP32.h:12-17
@@ -71,7 +71,7 @@ typedef enum psdp_method {
   PSDP_METHOD_COMPOSITE_FP32 = 0,          /* FP32 table, FP32 MFMA everywhere          */
   PSDP_METHOD_COMPOSITE_FP16 = 1,          /* FP16 table, FP16 operands everywhere      */
   PSDP_METHOD_COMPOSITE_FP32_EMULATED = 2, /* FP32 table, BF16x9 everywhere             */
-  PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 steps 0-2, FP16 after  */
+  PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 steps 0-3 + final      */
   PSDP_METHOD_CUSTOM = 4                   /* table + formats taken from psdp_config_t  */
 } psdp_method_t;
 
@@ -87,6 +87,7 @@ typedef struct psdp_config {
   double lanczos_tol;             /* default 1e-10 (lanczos.h:16)                    */
   double scale_margin;            /* s = up * margin; default 1.0 (reference)        */
   int synchronize;                /* 1: wait for the stream and check device flags   */
+  int average_products;           /* CUSTOM: 1-piece 16-bit steps use (C + C^T)/2    */
 } psdp_config_t;
 
 typedef struct psdp_handle_st* psdp_handle_t;
@@ -131,11 +132,12 @@ psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int6
 psdp_status_t psdp_project_f32(psdp_handle_t h, const float* X, float* Y, int64_t n,
                                const psdp_config_t* cfg, void* ws, size_t ws_bytes);
 
-/* One symmetric product C = X Y (X, Y symmetric n x n float32, column-major)
- * with operands rounded to `format`; C receives the float32 result.  Exposes
- * the filter-chain GEMM kernel on its own. */
+/* One symmetric product of X, Y (symmetric n x n float32, column-major) with
+ * operands rounded to `format`: C = upper triangle of X Y mirrored, or with
+ * average != 0 (binary16/bfloat16 only) C = (X Y + Y X)/2.  Exposes the
+ * filter-chain GEMM kernel on its own. */
 psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float* Y, float* C, int64_t n,
-                                   int format);
+                                   int format, int average);
 
 /* ---- profiling: HIP events around every launch of the handle, by kind ---- */
 typedef enum psdp_prof_kind {
@@ -157,6 +159,10 @@ typedef struct psdp_block {
 } psdp_block_t;
 psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
                                        const psdp_config_t* cfg);
+/* Workspace the batched call needs (host only: also builds and checks every
+ * launch descriptor of the call without touching a device). */
+psdp_status_t psdp_workspace_size_batched(const psdp_block_t* blocks, int count, const psdp_config_t* cfg,
+                                          size_t* bytes);
 
 /* ---- EVD projections (rocSOLVER), drop-ins of eig_FP64_psd / eig_FP32_psd ----
  * eigenvalues: NULL or a caller-provided device array of n doubles that

@@ -183,7 +183,26 @@ def _sym(m):
     return (m + m.T) * m.dtype.type(0.5)
 
 
-def composite_chain(X, coeffs, step_formats, final_format=None):
+def _mirror_upper(m):
+    """Upper triangle (with the diagonal) copied to the lower triangle."""
+    u = np.triu(m)
+    return u + np.triu(m, 1).T
+
+
+def sym_product(x, y, fmt, how="average"):
+    """Symmetric result of x @ y (x, y symmetric, commuting in exact
+    arithmetic).  how="average": (C + C^T)/2 of the full product;
+    how="mirror": the upper triangle of C mirrored (what a half-triangle
+    kernel writes); how="full": C itself."""
+    c = emulated_product(x, y, fmt)
+    if how == "mirror":
+        return _mirror_upper(c)
+    if how == "average":
+        return _sym(c)
+    return c
+
+
+def composite_chain(X, coeffs, step_formats, final_format=None, sym_mode="reference"):
     """Composite polynomial filter chain on an already scaled symmetric X.
 
     Op order follows src/composite_FP32.cu:89-156 (and the FP16 variant
@@ -195,22 +214,31 @@ def composite_chain(X, coeffs, step_formats, final_format=None):
     ``step_formats[i]`` is the operand format of step i's three products;
     "f64" runs the whole chain in float64 (the exact-arithmetic restatement),
     any other format keeps the iterate in float32 like the reference.
+    ``sym_mode`` "reference" symmetrizes as the reference does (A and W
+    averaged with their transposes, A2/A3 left as computed); "mirror"
+    symmetrizes every product by copying its upper triangle (half-triangle
+    products); "average" averages every product with its transpose.
     """
     exact = all(f == "f64" for f in step_formats) and final_format in (None, "f64")
     dt = np.float64 if exact else np.float32
     A = np.asarray(X, dtype=dt).copy()
     Xs = A.copy()
+    how = {"reference": "full", "mirror": "mirror", "average": "average"}[sym_mode]
+    prod = lambda x, y, f: sym_product(x, y, f, how).astype(dt)
     for (a, b, c), fmt in zip(coeffs, step_formats):
         a, b, c = dt(a), dt(b), dt(c)
-        A2 = emulated_product(A, A, fmt).astype(dt)
-        A3 = emulated_product(A2, A, fmt).astype(dt)
+        A2 = prod(A, A, fmt)
+        A3 = prod(A2, A, fmt)
         A = a * A + b * A3
-        A = A + c * emulated_product(A3, A2, fmt).astype(dt)
-        A = _sym(A)
+        A = A + c * prod(A3, A2, fmt)
+        if sym_mode == "reference":
+            A = _sym(A)
     n = A.shape[0]
     A = (A + np.eye(n, dtype=dt)) * dt(0.5)
-    W = emulated_product(Xs, A, final_format or step_formats[-1]).astype(dt)
-    return _sym(W).astype(np.float64)
+    W = prod(Xs, A, final_format or step_formats[-1])
+    if sym_mode == "reference":
+        W = _sym(W)
+    return W.astype(np.float64)
 
 
 def composite_FP32(X):
@@ -243,7 +271,7 @@ def composite_exact(X, table="fp32"):
 # Lanczos spectral-norm bound: src/lanczos.cu:41-293
 # ----------------------------------------------------------------------------
 
-def lanczos_two_norm(A, max_iter=20, tol=1e-10, q0=None, return_state=False):
+def lanczos_two_norm(A, max_iter=20, tol=1e-10, q0=None, return_state=False, keep_exact=True):
     """Restatement of approximate_two_norm (src/lanczos.cu:41-293).
 
     Lanczos on A^T A without reorthogonalisation.  Break rule of
@@ -282,6 +310,12 @@ def lanczos_two_norm(A, max_iter=20, tol=1e-10, q0=None, return_state=False):
             break
         v_old = q.copy()
         if beta[k] == 0.0:
+            # exact invariant subspace: the reference drops iteration k here
+            # (src/lanczos.cu:152-159), which turns e.g. any 1x1 matrix into
+            # the "zero matrix" case; with keep_exact it is kept (its Ritz
+            # value is exact) unless alpha_k == 0.
+            if keep_exact and alpha[k] > 0.0:
+                nb += 1
             break
         q = w * (1.0 / beta[k])
         if k < max_iter - 1:

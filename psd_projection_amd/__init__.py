@@ -70,6 +70,7 @@ class Config(ctypes.Structure):
         ("lanczos_tol", ctypes.c_double),
         ("scale_margin", ctypes.c_double),
         ("synchronize", ctypes.c_int),
+        ("average_products", ctypes.c_int),
     ]
 
 
@@ -103,9 +104,11 @@ _PROTOS = {
     "psdp_project_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_size_t]),
     "psdp_sym_product_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]),
+                                            ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "psdp_project_batched_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Block), ctypes.c_int,
                                                 ctypes.POINTER(Config)]),
+    "psdp_workspace_size_batched": (ctypes.c_int, [ctypes.POINTER(Block), ctypes.c_int, ctypes.POINTER(Config),
+                                                   ctypes.POINTER(ctypes.c_size_t)]),
     "psdp_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "psdp_profile_summary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_double)]),
@@ -359,16 +362,31 @@ def workspace_size(n, method="mixed", **overrides) -> int:
     return b.value
 
 
-def sym_product(handle, X, Y, fmt="f32", out=None):
-    """C = X Y of two symmetric float32 matrices with operands in `fmt`."""
+def sym_product(handle, X, Y, fmt="f32", out=None, average=False):
+    """Symmetric product of two symmetric float32 matrices with operands in
+    `fmt`: the upper triangle of X Y mirrored, or (X Y + Y X)/2 with average."""
     import torch
     handle._sync_stream()
     n = _size(X, None)
     if out is None:
         out = torch.empty_like(X)
     f = FORMATS[fmt] if isinstance(fmt, str) else int(fmt)
-    _check(lib().psdp_sym_product_f32(handle.ptr, _ptr(X), _ptr(Y), _ptr(out), n, f))
+    _check(lib().psdp_sym_product_f32(handle.ptr, _ptr(X), _ptr(Y), _ptr(out), n, f, 1 if average else 0))
     return out
+
+
+def workspace_size_batched(sizes, method="mixed", **overrides) -> int:
+    """Workspace bytes of a batched call on blocks of the given sizes (host
+    only; builds and checks the launch descriptors with stand-in pointers)."""
+    arr = (Block * len(sizes))()
+    for i, n in enumerate(sizes):
+        arr[i].X = 0x1000 + 16 * i
+        arr[i].Y = 0x2000 + 16 * i
+        arr[i].n = int(n)
+    cfg = default_config(method, **overrides)
+    b = ctypes.c_size_t()
+    _check(lib().psdp_workspace_size_batched(arr, len(sizes), ctypes.byref(cfg), ctypes.byref(b)))
+    return b.value
 
 
 def project_batched(handle, blocks, outs=None, method="mixed", config: Config | None = None, **overrides):

@@ -41,18 +41,22 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     srcs = sorted(CSRC.glob("*.hip"))
     objs = []
 
+    usage = {}
+
     def compile_one(src: Path) -> Path:
         obj = OBJ / (src.stem + ".o")
-        cmd = [hipcc, *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+        cmd = [hipcc, *HIPCC_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
-        msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
-                         if "packed-fp32-ops" not in l)
+        lines = (r.stdout + r.stderr).splitlines()
+        msgs = "\n".join(l for l in lines if "packed-fp32-ops" not in l and "remark:" not in l
+                         and "kernel-resource-usage" not in l)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed on {src.name}:\n{msgs}")
         if verbose and msgs.strip():
             print(msgs, flush=True)
+        usage[src.name] = _parse_usage(lines)
         return obj
 
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
@@ -66,7 +70,36 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    report = OBJ / "kernel_resources.txt"
+    with open(report, "w") as f:
+        for src, kernels in sorted(usage.items()):
+            for name, info in kernels.items():
+                f.write(f"{src} {name} " + " ".join(f"{k}={v}" for k, v in info.items()) + "\n")
+    spills = [(src, k) for src, ks in usage.items() for k, v in ks.items() if v.get("scratch", 0) > 0]
+    if spills:
+        raise RuntimeError(f"kernels use scratch memory (register spills or private arrays): {spills}")
     return LIB
+
+
+def _parse_usage(lines):
+    """Per-kernel VGPR/AGPR/scratch/LDS figures from -Rpass-analysis remarks."""
+    out, cur = {}, None
+    keys = {"VGPRs:": "vgpr", "AGPRs:": "agpr", "ScratchSize [bytes/lane]:": "scratch",
+            "LDS Size [bytes/block]:": "lds", "Occupancy [waves/SIMD]:": "occupancy"}
+    for l in lines:
+        if "Function Name:" in l:
+            cur = l.split("Function Name:")[1].split("[-Rpass")[0].strip()
+            out[cur] = {}
+            continue
+        if cur is None:
+            continue
+        for k, name in keys.items():
+            if k in l:
+                try:
+                    out[cur][name] = int(l.split(k)[1].split("[-Rpass")[0].strip())
+                except ValueError:
+                    pass
+    return out
 
 
 if __name__ == "__main__":

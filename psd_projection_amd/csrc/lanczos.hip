@@ -5,7 +5,9 @@
 // graph capturable):
 //   * Lanczos on A^T A = A A (A symmetric), no reorthogonalisation
 //     (src/lanczos.cu:116-169), the break rule of :145 and :152-159 evaluated
-//     on the device; after a break the remaining launches are no-ops;
+//     on the device (except that an exact invariant subspace keeps its
+//     iteration, see step_body); after a break the remaining launches are
+//     no-ops;
 //   * T = tridiag(alpha, beta) (src/lanczos.cu:16-38), its largest eigenpair
 //     by a one-wave cyclic Jacobi solver in FP64 (replaces cusolverDnDsyevd
 //     at :206-214; T is built from zeros, unlike the unzeroed buffer there);
@@ -15,32 +17,35 @@
 //     (:171-184).
 // The matrix-vector products are a bandwidth-bound GEMV (one wave streams
 // RPW rows with 16-byte loads, FP64 accumulation, wave-shuffle reduction);
-// vector updates run in one 1024-thread workgroup per matrix.
+// vector updates run in one 1024-thread workgroup per matrix.  Every kernel
+// has a single-matrix form and a batched form (one launch for many matrices,
+// blockIdx.y or blockIdx.x selects the matrix) sharing the same body, so a
+// matrix gets bitwise the same bound alone and in a batch.
 #include "psdp_device.h"
 
 namespace psdp {
 
+constexpr int GEMV_RPW = 4;                    // rows per wave
+constexpr int GEMV_ROWS_PER_BLOCK = 4 * GEMV_RPW;
+
 // ------------------------------------------------------------------ GEMV
-// y[r] = sum_k A[r*ld + k] x[k] for r < m (A symmetric: a row of the
-// row-major view is a column of the column-major matrix).
-template <typename TA, int VEC, int RPW>
-__global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A, int64_t ld, int m,
-                                                       const double* __restrict__ x,
-                                                       double* __restrict__ y,
-                                                       const int* __restrict__ skip) {
-  if (skip && *skip) return;
+// y[r] = sum_k A[r*ld + k] x[k] for the rows of this workgroup (A symmetric:
+// a row of the row-major view is a column of the column-major matrix).
+template <typename TA, int VEC>
+__device__ __forceinline__ void gemv_body(const TA* __restrict__ A, int64_t ld, int m, const double* __restrict__ x,
+                                          double* __restrict__ y, int block) {
   const int lane = threadIdx.x & 63;
-  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+  const int r0 = (block * 4 + (threadIdx.x >> 6)) * GEMV_RPW;
   if (r0 >= m) return;
-  double acc[RPW];
+  double acc[GEMV_RPW];
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) acc[r] = 0.0;
+  for (int r = 0; r < GEMV_RPW; ++r) acc[r] = 0.0;
   for (int k = lane * VEC; k < m; k += 64 * VEC) {
     double xv[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) xv[e] = x[k + e];
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
+    for (int r = 0; r < GEMV_RPW; ++r) {
       if (r0 + r < m) {
         const TA* row = A + (int64_t)(r0 + r) * ld + k;
         TA av[VEC];
@@ -56,10 +61,28 @@ __global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A,
     }
   }
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) {
+  for (int r = 0; r < GEMV_RPW; ++r) {
     const double s = wave_sum(acc[r]);
     if (lane == 0 && r0 + r < m) y[r0 + r] = s;
   }
+}
+
+template <typename TA, int VEC>
+__global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A, int64_t ld, int m,
+                                                       const double* __restrict__ x, double* __restrict__ y,
+                                                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  gemv_body<TA, VEC>(A, ld, m, x, y, blockIdx.x);
+}
+
+// which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y (skip unless Ritz pair)
+__global__ __launch_bounds__(256) void gemv_batched_kernel(const LanczosJob* __restrict__ jobs, int which) {
+  const LanczosJob& J = jobs[blockIdx.y];
+  if (which < 2 ? J.st->done : J.st->skip_ritz) return;
+  if ((int)blockIdx.x * GEMV_ROWS_PER_BLOCK >= J.m) return;
+  const double* x = which == 0 ? J.v.q : (which == 1 ? J.v.w1 : J.v.y);
+  double* y = which == 0 ? J.v.w1 : (which == 1 ? J.v.w : J.v.ay);
+  gemv_body<float, 4>(J.A, J.ld, J.m, x, y, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ start
@@ -73,14 +96,14 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
 // q_i ~ U(0,1] for i < n (zero in the padding), q <- q/||q||, V(:,0) = q,
 // vold = 0 (src/lanczos.cu:69,101-112).  The start vector is a seeded
 // counter-based stream instead of cuRAND XORWOW with a clock seed.
-__global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, LanczosVecs v, int n, int m,
-                                                            uint64_t seed) {
+__device__ void init_body(LanczosState* st, const LanczosVecs& v, int n, int m, uint64_t seed) {
   __shared__ double red[16];
   double s = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
     double qi = 0.0;
-    if (i < n) qi = ((double)(splitmix64(seed ^ (0x632be59bd9b4e019ull * (uint64_t)(i + 1))) >> 11) + 1.0) *
-                    (1.0 / 9007199254740992.0);
+    if (i < n)
+      qi = ((double)(splitmix64(seed ^ (0x632be59bd9b4e019ull * (uint64_t)(i + 1))) >> 11) + 1.0) *
+           (1.0 / 9007199254740992.0);
     v.q[i] = qi;
     s += qi * qi;
   }
@@ -108,10 +131,9 @@ __global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, La
 }
 
 // One Lanczos iteration after w = A (A q) (src/lanczos.cu:130-168).
-__global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int m,
-                                                            int max_iter, double tol) {
+__device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int m, int max_iter, double tol) {
   __shared__ double red[16];
-  if (st->done) return;
+  if (st->done || k >= max_iter) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) s += v.q[i] * v.w[i];
   const double alpha = block_sum(s, red);
@@ -125,7 +147,14 @@ __global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, La
   }
   const double beta = sqrt(block_sum(s, red));
   const bool bad = !isfinite(alpha) || !isfinite(beta);
-  const bool stop = bad || (beta <= tol * alpha && k > 1) || beta == 0.0;
+  // src/lanczos.cu:145 drops iteration k on convergence.  On an exact
+  // invariant subspace (beta == 0, src/lanczos.cu:152-159) the reference also
+  // drops it, so a start vector that is an eigenvector (any 1x1 matrix) is
+  // taken for the zero matrix and gets up = 1; here that iteration is kept
+  // (its Ritz value is exact) unless alpha == 0 (the zero matrix itself).
+  const bool drop = bad || (beta <= tol * alpha && k > 1) || (beta == 0.0 && !(alpha > 0.0));
+  const bool exact = !drop && beta == 0.0;
+  const bool stop = drop || exact;
   if (!stop) {
     const double inv = 1.0 / beta;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
@@ -140,7 +169,10 @@ __global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, La
     st->alpha[k] = alpha;
     st->beta[k] = beta;
     if (bad) st->nonfinite = 1;
-    if (stop) {
+    if (drop) {
+      st->done = 1;
+    } else if (exact) {
+      st->nb = k + 1;
       st->done = 1;
     } else {
       st->beta_old = beta;
@@ -152,7 +184,7 @@ __global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, La
 
 // Largest eigenpair of T = tridiag(alpha[0..nb), beta[0..nb-1)) by cyclic
 // Jacobi rotations in FP64, one wave (lane k owns row/column k).
-__global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) {
+__device__ void ritz_body(LanczosState* st) {
   __shared__ double T[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
   __shared__ double Z[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
   const int lane = threadIdx.x;
@@ -201,7 +233,10 @@ __global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) {
         const double app = T[p][p], aqq = T[q][q];
         if (fabs(apq) <= 1e-300 || fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq))) {
           __syncthreads();
-          if (lane == 0) { T[p][q] = 0.0; T[q][p] = 0.0; }
+          if (lane == 0) {
+            T[p][q] = 0.0;
+            T[q][p] = 0.0;
+          }
           __syncthreads();
           continue;
         }
@@ -239,7 +274,6 @@ __global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) {
       }
     }
   }
-  // largest eigenvalue and its vector
   double best = -1e308;
   int bi = 0;
   for (int c = 0; c < nb; ++c) {
@@ -254,11 +288,9 @@ __global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) {
 }
 
 // y = V(:, 0:nb) u  (src/lanczos.cu:242)
-__global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st, LanczosVecs v, int m) {
-  if (st->skip_ritz) return;
+__device__ __forceinline__ void ritz_vector_body(const LanczosState* st, const LanczosVecs& v, int m, int i) {
+  if (st->skip_ritz || i >= m) return;
   const int nb = st->nb;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
   double s = 0.0;
   for (int j = 0; j < nb; ++j) s = fma(v.V[(int64_t)j * m + i], st->u[j], s);
   v.y[i] = s;
@@ -266,8 +298,7 @@ __global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st
 
 // r = A y - theta y; lo = sqrt(theta); up = sqrt(theta + ||r||)
 // (src/lanczos.cu:258-270); s = up > 0 ? up : 1 (src/composite_FP32.cu:182).
-__global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, LanczosVecs v, int m,
-                                                             double margin) {
+__device__ void bound_body(LanczosState* st, const LanczosVecs& v, int m, double margin) {
   __shared__ double red[16];
   if (st->skip_ritz) return;
   const double th = st->theta;
@@ -278,15 +309,54 @@ __global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, L
   }
   const double rn = sqrt(block_sum(s, red));
   if (threadIdx.x == 0) {
-    const double up = sqrt(th + rn);
+    // theta >= 0 in exact arithmetic (Ritz value of A^T A); clamp rounding
+    const double thc = th > 0.0 ? th : 0.0;
+    const double up = sqrt(thc + rn);
     st->rnorm = rn;
-    st->lo = sqrt(th);
+    st->lo = sqrt(thc);
     st->up = up;
     const double sc = up > 0.0 ? up * margin : 1.0;
     st->scale = sc;
     st->inv_scale = 1.0 / sc;
     if (!isfinite(up)) st->nonfinite = 1;
   }
+}
+
+// ------------------------------------------------------------------ single
+__global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, LanczosVecs v, int n, int m,
+                                                            uint64_t seed) {
+  init_body(st, v, n, m, seed);
+}
+__global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int m,
+                                                            int max_iter, double tol) {
+  step_body(st, v, k, m, max_iter, tol);
+}
+__global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) { ritz_body(st); }
+__global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st, LanczosVecs v, int m) {
+  ritz_vector_body(st, v, m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, LanczosVecs v, int m,
+                                                             double margin) {
+  bound_body(st, v, m, margin);
+}
+
+// ------------------------------------------------------------------ batched
+__global__ __launch_bounds__(1024) void lanczos_init_batched(const LanczosJob* jobs, uint64_t seed) {
+  const LanczosJob& J = jobs[blockIdx.x];
+  init_body(J.st, J.v, J.n, J.m, seed);
+}
+__global__ __launch_bounds__(1024) void lanczos_step_batched(const LanczosJob* jobs, int k, double tol) {
+  const LanczosJob& J = jobs[blockIdx.x];
+  step_body(J.st, J.v, k, J.m, J.max_iter, tol);
+}
+__global__ __launch_bounds__(64) void lanczos_ritz_batched(const LanczosJob* jobs) { ritz_body(jobs[blockIdx.x].st); }
+__global__ __launch_bounds__(256) void ritz_vector_batched(const LanczosJob* jobs) {
+  const LanczosJob& J = jobs[blockIdx.y];
+  ritz_vector_body(J.st, J.v, J.m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ __launch_bounds__(1024) void lanczos_bound_batched(const LanczosJob* jobs, double margin) {
+  const LanczosJob& J = jobs[blockIdx.x];
+  bound_body(J.st, J.v, J.m, margin);
 }
 
 // Fixed scale (no bound): used by the non-auto-scaled entry points.
@@ -301,15 +371,13 @@ __global__ void set_scale_kernel(LanczosState* st, double scale) {
 template <typename TA>
 static void launch_gemv(const TA* A, int64_t ld, int m, const double* x, double* y, const int* skip,
                         hipStream_t s) {
-  constexpr int RPW = 4;
-  const int rows_per_block = 4 * RPW;
-  const int grid = (m + rows_per_block - 1) / rows_per_block;
+  const int grid = (m + GEMV_ROWS_PER_BLOCK - 1) / GEMV_ROWS_PER_BLOCK;
   constexpr int VEC = 16 / (int)sizeof(TA);
   const bool aligned = (m % VEC == 0) && (ld % VEC == 0) && (((uintptr_t)A) % 16 == 0);
   if (aligned)
-    hipLaunchKernelGGL((gemv_sym_kernel<TA, VEC, RPW>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
+    hipLaunchKernelGGL((gemv_sym_kernel<TA, VEC>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
   else
-    hipLaunchKernelGGL((gemv_sym_kernel<TA, 1, RPW>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
+    hipLaunchKernelGGL((gemv_sym_kernel<TA, 1>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
 }
 
 // Enqueues the whole bound.  A: symmetric m x m (row-major view = column-major),
@@ -335,6 +403,24 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
   else
     launch_gemv((const float*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
   hipLaunchKernelGGL(lanczos_bound_kernel, dim3(1), dim3(1024), 0, s, st, v, m, margin);
+  return hipGetLastError();
+}
+
+// Batched bound over `count` float32 padded matrices (jobs in device memory;
+// every job's m is a multiple of 4 and at most max_m).
+hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
+                                  uint64_t seed, double margin, hipStream_t s) {
+  const dim3 ggrid((max_m + GEMV_ROWS_PER_BLOCK - 1) / GEMV_ROWS_PER_BLOCK, count);
+  hipLaunchKernelGGL(lanczos_init_batched, dim3(count), dim3(1024), 0, s, jobs, seed);
+  for (int k = 0; k < max_iter; ++k) {
+    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 0);
+    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 1);
+    hipLaunchKernelGGL(lanczos_step_batched, dim3(count), dim3(1024), 0, s, jobs, k, tol);
+  }
+  hipLaunchKernelGGL(lanczos_ritz_batched, dim3(count), dim3(64), 0, s, jobs);
+  hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 2);
+  hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin);
   return hipGetLastError();
 }
 

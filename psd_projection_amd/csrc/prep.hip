@@ -92,6 +92,66 @@ __global__ __launch_bounds__(256) void scale_split_kernel(SplitArgs s) {
   }
 }
 
+// Batched form (float32 inputs): blockIdx.z selects the matrix; tiles beyond
+// that matrix's padded size exit at once.
+__global__ __launch_bounds__(256) void sym_pad_batched_kernel(const PadJob* __restrict__ jobs) {
+  const PadJob& J = jobs[blockIdx.z];
+  const int T = J.npad / 64;
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bi >= T || bj >= T || bi > bj) return;
+  __shared__ float s1[64][65];
+  __shared__ float s2[64][65];
+  const int t = threadIdx.x;
+  const int r = t & 63;
+  const int i0 = bi * 64, j0 = bj * 64;
+  const int n = J.n;
+  const int64_t ld = J.npad;
+#pragma unroll 4
+  for (int c = t >> 6; c < 64; c += 4) {
+    const int gi = i0 + r, gj = j0 + c;
+    s1[c][r] = (gi < n && gj < n) ? J.X[(int64_t)gi + (int64_t)gj * n] : 0.0f;
+    const int hi = j0 + r, hj = i0 + c;
+    s2[c][r] = (hi < n && hj < n) ? J.X[(int64_t)hi + (int64_t)hj * n] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int a = t >> 6; a < 64; a += 4) {
+    const int b = r;
+    J.M[(int64_t)(i0 + a) * ld + j0 + b] = 0.5f * (s1[b][a] + s2[a][b]);
+    if (bi != bj) J.M[(int64_t)(j0 + a) * ld + i0 + b] = 0.5f * (s2[b][a] + s1[a][b]);
+  }
+}
+
+__global__ __launch_bounds__(256) void scale_split_batched_kernel(const SplitJob* __restrict__ jobs, int fmt_a,
+                                                                  int fmt_x) {
+  const SplitJob& J = jobs[blockIdx.y];
+  const double inv = J.inv ? *J.inv : 1.0;
+  uint16_t* ap[3] = {J.ap[0], J.ap[1], J.ap[2]};
+  uint16_t* xp[3] = {J.xp[0], J.xp[1], J.xp[2]};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < J.count; i += stride) {
+    const float v = (float)((double)J.a32[i] * inv);
+    J.a32[i] = v;
+    if (fmt_a != FMT_F32) store_operand(fmt_a, v, i, nullptr, ap);
+    store_operand(fmt_x, v, i, J.x32, xp);
+  }
+}
+
+hipError_t launch_sym_pad_batched(const PadJob* jobs, int count, int max_npad, hipStream_t st) {
+  const int T = max_npad / 64;
+  hipLaunchKernelGGL(sym_pad_batched_kernel, dim3(T, T, count), dim3(256), 0, st, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_split_batched(const SplitJob* jobs, int count, int64_t max_count, int fmt_a, int fmt_x,
+                                      hipStream_t st) {
+  int64_t blocks = (max_count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(scale_split_batched_kernel, dim3((unsigned)blocks, count), dim3(256), 0, st, jobs, fmt_a,
+                     fmt_x);
+  return hipGetLastError();
+}
+
 hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
                           hipStream_t st) {
   const int T = npad / 64;

@@ -25,9 +25,9 @@
 #include "psdp_internal.h"
 
 namespace psdp {
-hipError_t launch_sym_product(int fmt, int sub, const ProdDesc& d, const TileRef* tiles, int ntiles,
+hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, const TileRef* tiles, int ntiles,
                               hipStream_t s);
-hipError_t launch_sym_product_grouped(int fmt, int sub, const ProdDesc* descs, const TileRef* tiles,
+hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,
                                       int ntiles, int grid, hipStream_t s);
 hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
                           hipStream_t st);
@@ -37,6 +37,15 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
                           uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
                           hipStream_t s);
 hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s);
+hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
+                                  uint64_t seed, double margin, hipStream_t s);
+hipError_t launch_sym_pad_batched(const PadJob* jobs, int count, int max_npad, hipStream_t st);
+hipError_t launch_scale_split_batched(const SplitJob* jobs, int count, int64_t max_count, int fmt_a, int fmt_x,
+                                      hipStream_t st);
+size_t eig_workspace_bytes(int n, int f64);
+int eig_project(void* rb_handle, hipStream_t s, double* X, int n, int f64, void* ws, int* info_dev, double* eig);
+void* eig_create_blas();
+void eig_destroy_blas(void* rb);
 }  // namespace psdp
 
 using namespace psdp;
@@ -87,8 +96,13 @@ struct Schedule {
   int nsteps = 0;
   float a[PSDP_MAX_STEPS], b[PSDP_MAX_STEPS], c[PSDP_MAX_STEPS];
   int fmt[PSDP_MAX_STEPS];
+  int avg[PSDP_MAX_STEPS];  // A^3 and A^5 products averaged with their transposes
   int final_fmt = FMT_F32;
+  int final_avg = 0;
 };
+
+// The averaged (Jordan) product exists for the 1-piece 16-bit formats.
+static int avg_capable(int fmt) { return fmt == FMT_F16 || fmt == FMT_BF16; }
 
 // Coefficients as the reference computes them, in float arithmetic:
 // a /= sf; b /= sf*sf*sf; c /= sf*sf*sf*sf*sf on the scaled steps
@@ -113,7 +127,15 @@ static void fill_table(int table, Schedule& s) {
   }
 }
 
+// The reference symmetrizes A once per step by averaging with its transpose
+// (src/utils.cu:7-32, called at src/composite_FP32.cu:118); with binary16
+// operands the non-square products A2 A and A3 A2 are visibly non-symmetric,
+// so the FP16 method averages them (C + C^T)/2 to keep the reference's error.
+// Other formats keep the upper triangle (their commutator error is below the
+// float32 accumulation error).
 static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
+  for (int i = 0; i < PSDP_MAX_STEPS; ++i) s.avg[i] = 0;
+  s.final_avg = 0;
   switch (cfg->method) {
     case PSDP_METHOD_COMPOSITE_FP32:
       fill_table(PSDP_TABLE_FP32, s);
@@ -122,8 +144,9 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
       break;
     case PSDP_METHOD_COMPOSITE_FP16:
       fill_table(PSDP_TABLE_FP16, s);
-      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = FMT_F16;
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = FMT_F16, s.avg[i] = 1;
       s.final_fmt = FMT_F16;
+      s.final_avg = 1;
       break;
     case PSDP_METHOD_COMPOSITE_FP32_EMULATED:
       fill_table(PSDP_TABLE_FP32, s);
@@ -131,9 +154,12 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
       s.final_fmt = FMT_BF16X9;
       break;
     case PSDP_METHOD_MIXED:
+      // FP16-table chain: hi/lo split operands on the first 4 steps, where
+      // rounding errors are amplified by the later steps, plain binary16 on
+      // the last 3, split operands again for the final product.
       fill_table(PSDP_TABLE_FP16, s);
-      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 3 ? FMT_F16X3 : FMT_F16;
-      s.final_fmt = FMT_F16;
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 4 ? FMT_F16X3 : FMT_F16;
+      s.final_fmt = FMT_F16X3;
       break;
     case PSDP_METHOD_CUSTOM:
       if (cfg->table != PSDP_TABLE_FP32 && cfg->table != PSDP_TABLE_FP16)
@@ -143,10 +169,12 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
         if (cfg->step_format[i] < 0 || cfg->step_format[i] >= FMT_COUNT)
           return fail(PSDP_ERR_INVALID_ARG, "step %d: unknown format %d", i, cfg->step_format[i]);
         s.fmt[i] = cfg->step_format[i];
+        s.avg[i] = cfg->average_products && avg_capable(s.fmt[i]);
       }
       if (cfg->final_format < 0 || cfg->final_format >= FMT_COUNT)
         return fail(PSDP_ERR_INVALID_ARG, "unknown final format %d", cfg->final_format);
       s.final_fmt = cfg->final_format;
+      s.final_avg = cfg->average_products && avg_capable(s.final_fmt);
       break;
     default:
       return fail(PSDP_ERR_INVALID_ARG, "unknown method %d", cfg->method);
@@ -179,6 +207,7 @@ struct psdp_handle_st {
   size_t arena_bytes = 0;
   LanczosState* st = nullptr;  // device
   uint64_t seed = 0x5eed5eedULL;
+  void* blas = nullptr;  // rocBLAS handle for the EVD drop-ins (created on first use)
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
   std::mutex mu;
 };
@@ -265,14 +294,53 @@ struct ProfScope {
   }
 };
 
+static constexpr size_t kNone = ~(size_t)0;
+
+// Host-side check of a product descriptor before it is launched: every
+// operand piece the format reads and every buffer the epilogue writes must be
+// present, so that a planning mistake becomes an error instead of a fault.
+static bool desc_ok(const ProdDesc& d, int fmt, const char** why) {
+  const int np = fmt == FMT_F32 ? 1 : (fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : 1));
+  for (int i = 0; i < np; ++i)
+    if (!d.x[i] || !d.y[i]) return *why = "missing operand piece", false;
+  if (d.npad <= 0 || d.npad % 64 || d.ld < d.npad || d.n > d.npad) return *why = "bad sizes", false;
+  switch (d.mode) {
+    case EPI_STORE:
+      if (!d.c32 && d.out_fmt == FMT_F32) return *why = "no float32 destination", false;
+      break;
+    case EPI_POLY:
+    case EPI_POLY_HALF_ID:
+      if (!d.c32 || !d.p3) return *why = "missing A or A^3", false;
+      break;
+    case EPI_OUTPUT:
+      if (!d.out || d.out_ld < d.n) return *why = "missing output", false;
+      return true;
+    default:
+      return *why = "bad mode", false;
+  }
+  if (d.out_fmt != FMT_F32) {
+    const int no = d.out_fmt == FMT_F16X3 || d.out_fmt == FMT_BF16X3 ? 2 : (d.out_fmt == FMT_BF16X9 ? 3 : 1);
+    for (int i = 0; i < no; ++i)
+      if (!d.outp[i]) return *why = "missing output piece", false;
+  }
+  return true;
+}
+
+#define DESCCHK(d, fmt)                                                                   \
+  do {                                                                                    \
+    const char* why_ = "";                                                                \
+    if (!desc_ok((d), (fmt), &why_)) return fail(PSDP_ERR_INVALID_ARG, "internal: %s", why_); \
+  } while (0)
+
 // ---------------------------------------------------------------- plan
 struct Plan {
   int n = 0, npad = 0, sub = 4, bt = 128;
   int64_t N = 0;
   Schedule sch;
   int lanczos_iters = 20;
-  size_t off_a32 = 0, off_p3 = 0, off_p2 = 0, off_x32 = 0;
-  size_t off_ap[3] = {0, 0, 0}, off_p2p[3] = {0, 0, 0}, off_p3p[3] = {0, 0, 0}, off_xp[3] = {0, 0, 0};
+  size_t off_a32 = 0, off_p3 = 0, off_p2 = kNone, off_x32 = kNone;
+  size_t off_ap[3] = {kNone, kNone, kNone}, off_p2p[3] = {kNone, kNone, kNone};
+  size_t off_p3p[3] = {kNone, kNone, kNone}, off_xp[3] = {kNone, kNone, kNone};
   size_t off_vec = 0;
   size_t total = 0;
 };
@@ -327,13 +395,13 @@ static Bufs carve(const Plan& p, char* base) {
   Bufs b;
   b.a32 = (float*)(base + p.off_a32);
   b.p3 = (float*)(base + p.off_p3);
-  b.p2 = p.off_p2 ? (float*)(base + p.off_p2) : nullptr;
-  b.x32 = p.off_x32 ? (float*)(base + p.off_x32) : nullptr;
+  b.p2 = p.off_p2 != kNone ? (float*)(base + p.off_p2) : nullptr;
+  b.x32 = p.off_x32 != kNone ? (float*)(base + p.off_x32) : nullptr;
   for (int i = 0; i < 3; ++i) {
-    b.ap[i] = p.off_ap[i] ? (uint16_t*)(base + p.off_ap[i]) : nullptr;
-    b.p2p[i] = p.off_p2p[i] ? (uint16_t*)(base + p.off_p2p[i]) : nullptr;
-    b.p3p[i] = p.off_p3p[i] ? (uint16_t*)(base + p.off_p3p[i]) : nullptr;
-    b.xp[i] = p.off_xp[i] ? (uint16_t*)(base + p.off_xp[i]) : nullptr;
+    b.ap[i] = p.off_ap[i] != kNone ? (uint16_t*)(base + p.off_ap[i]) : nullptr;
+    b.p2p[i] = p.off_p2p[i] != kNone ? (uint16_t*)(base + p.off_p2p[i]) : nullptr;
+    b.p3p[i] = p.off_p3p[i] != kNone ? (uint16_t*)(base + p.off_p3p[i]) : nullptr;
+    b.xp[i] = p.off_xp[i] != kNone ? (uint16_t*)(base + p.off_xp[i]) : nullptr;
   }
   const size_t vb = align256((size_t)p.npad * 8);
   char* v = base + p.off_vec;
@@ -404,8 +472,9 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d1.out_fmt = f;
     for (int q = 0; q < 3; ++q) d1.outp[q] = b.p2p[q];
     {
+      DESCCHK(d1, f);
       ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, d1, tl.dev, tl.count, s));
+      HIPCHK(launch_sym_product(f, p.sub, 0, d1, tl.dev, tl.count, s));
     }
     // A3 = A2 A
     ProdDesc d2 = base_desc(p);
@@ -416,8 +485,9 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d2.out_fmt = f;
     for (int q = 0; q < 3; ++q) d2.outp[q] = b.p3p[q];
     {
+      DESCCHK(d2, f);
       ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, d2, tl.dev, tl.count, s));
+      HIPCHK(launch_sym_product(f, p.sub, sch.avg[i], d2, tl.dev, tl.count, s));
     }
     // A = a A + b A3 + c A3 A2 (last step: S = (I + A)/2)
     ProdDesc d3 = base_desc(p);
@@ -432,8 +502,9 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d3.out_fmt = fn;
     for (int q = 0; q < 3; ++q) d3.outp[q] = b.ap[q];
     {
+      DESCCHK(d3, f);
       ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, d3, tl.dev, tl.count, s));
+      HIPCHK(launch_sym_product(f, p.sub, sch.avg[i], d3, tl.dev, tl.count, s));
     }
   }
   // W = X_s S, Y = s W
@@ -446,8 +517,9 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   d4.out_f64 = y_f64;
   d4.scale = &h->st->scale;
   {
+    DESCCHK(d4, sch.final_fmt);
     ProfScope ps(h, PSDP_PROF_PROD_F32 + sch.final_fmt);
-    HIPCHK(launch_sym_product(sch.final_fmt, p.sub, d4, tl.dev, tl.count, s));
+    HIPCHK(launch_sym_product(sch.final_fmt, p.sub, sch.final_avg, d4, tl.dev, tl.count, s));
   }
   return PSDP_OK;
 }
@@ -586,6 +658,7 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   for (auto& kv : h->tiles) (void)hipFree(kv.second.dev);
   if (h->arena) (void)hipFree(h->arena);
   if (h->st) (void)hipFree(h->st);
+  eig_destroy_blas(h->blas);
   if (h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return PSDP_OK;
@@ -718,10 +791,12 @@ psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_
 }
 
 psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float* Y, float* C, int64_t n,
-                                   int format) {
+                                   int format, int average) {
   if (!h || !X || !Y || !C) return fail(PSDP_ERR_INVALID_ARG, "null argument");
   if (n <= 0 || n > (1 << 20)) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size");
   if (format < 0 || format >= FMT_COUNT) return fail(PSDP_ERR_INVALID_ARG, "unknown format %d", format);
+  if (average && !avg_capable(format))
+    return fail(PSDP_ERR_UNSUPPORTED, "averaged products exist for binary16 and bfloat16 operands only");
   std::lock_guard<std::mutex> lock(h->mu);
   HIPCHK(hipSetDevice(h->device));
   const int sub = choose_sub(n), bt = 32 * sub;
@@ -759,34 +834,325 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   d.out_ld = n;
   d.out_f64 = 0;
   d.scale = nullptr;
-  HIPCHK(launch_sym_product(format, sub, d, tl.dev, tl.count, s));
+  HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
   HIPCHK(hipStreamSynchronize(s));
+  return PSDP_OK;
+}
+
+// Many independent blocks (a block-diagonal cone): every block keeps its own
+// bound, scale and buffers; each stage of the chain is ONE persistent grouped
+// launch over the upper-triangle tiles of all blocks (64 x 64 tiles, sorted by
+// K length so that the static round-robin over workgroups stays balanced).
+// A block's result is bitwise the same alone and in any batch: the same tile
+// code, k order and bound kernels run on it either way.
+struct BatchPlan {
+  Schedule sch;
+  int iters = 20;
+  double tol = 1e-10, margin = 1.0;
+  int sub = 2, bt = 64, count = 0, nstages = 0, ntiles = 0, max_npad = 0;
+  int64_t max_N = 0;
+  size_t off_states = 0, off_desc = 0, total = 0;
+  size_t d_lz = 0, d_pad = 0, d_split = 0, d_prod = 0, d_tiles = 0;
+  std::vector<char> blob;  // descriptors, built for one workspace base
+};
+
+// Lays out the workspace and, when base is given, builds every descriptor for
+// that base and checks them.  Pure host code: no device call.
+static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const psdp_config_t* cfg, char* base,
+                                  BatchPlan& bp) {
+  if (!blocks) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (count <= 0 || count > 65535) return fail(PSDP_ERR_INVALID_ARG, "block count must be in [1, 65535]");
+  Schedule& sch = bp.sch;
+  STCHK(build_schedule(cfg, sch));
+  bp.iters = cfg->lanczos_iters > 0 ? cfg->lanczos_iters : 20;
+  if (bp.iters > LANCZOS_MAX_ITER)
+    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
+  bp.tol = cfg->lanczos_tol > 0 ? cfg->lanczos_tol : 1e-10;
+  bp.margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
+  for (int b = 0; b < count; ++b)
+    if (!blocks[b].X || !blocks[b].Y || blocks[b].n <= 0 || blocks[b].n > 65536)
+      return fail(PSDP_ERR_INVALID_ARG, "block %d: bad pointer or size", b);
+  bp.count = count;
+  const int bt = bp.bt;
+  bool any_f32 = false;
+  int pa = pieces_of(sch.final_fmt), pp = 0;
+  for (int i = 0; i < sch.nsteps; ++i) {
+    any_f32 |= sch.fmt[i] == FMT_F32;
+    pa = std::max(pa, pieces_of(sch.fmt[i]));
+    pp = std::max(pp, pieces_of(sch.fmt[i]));
+  }
+  const int px = pieces_of(sch.final_fmt);
+
+  struct BlockBufs {
+    int n, npad, iters;
+    size_t a32, p3, p2, x32, ap[3], p2p[3], p3p[3], xp[3], vec;
+  };
+  std::vector<BlockBufs> bb(count);
+  size_t off = 0;
+  for (int b = 0; b < count; ++b) {
+    BlockBufs& q = bb[b];
+    // kNone marks buffers this schedule does not use (offset 0 is a real buffer)
+    q.p2 = q.x32 = kNone;
+    for (int i = 0; i < 3; ++i) q.ap[i] = q.p2p[i] = q.p3p[i] = q.xp[i] = kNone;
+    q.n = (int)blocks[b].n;
+    q.npad = (q.n + bt - 1) / bt * bt;
+    q.iters = std::min(bp.iters, q.n);
+    bp.max_npad = std::max(bp.max_npad, q.npad);
+    const int64_t N = (int64_t)q.npad * q.npad;
+    bp.max_N = std::max(bp.max_N, N);
+    const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
+    q.a32 = off, off += f32b;
+    q.p3 = off, off += f32b;
+    if (any_f32) q.p2 = off, off += f32b;
+    if (sch.final_fmt == FMT_F32) q.x32 = off, off += f32b;
+    for (int i = 0; i < pa; ++i) q.ap[i] = off, off += u16b;
+    for (int i = 0; i < pp; ++i) q.p2p[i] = off, off += u16b;
+    for (int i = 0; i < pp; ++i) q.p3p[i] = off, off += u16b;
+    for (int i = 0; i < px; ++i) q.xp[i] = off, off += u16b;
+    q.vec = off;
+    off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
+  }
+  std::vector<TileRef> tiles;
+  for (int b = 0; b < count; ++b) {
+    const int T = bb[b].npad / bt;
+    for (int i = 0; i < T; ++i)
+      for (int j = i; j < T; ++j) tiles.push_back(TileRef{b, i, j, 0});
+  }
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [&](const TileRef& x, const TileRef& y) { return bb[x.mat].npad > bb[y.mat].npad; });
+  bp.ntiles = (int)tiles.size();
+  bp.nstages = 3 * sch.nsteps + 1;
+  bp.off_states = off;
+  off += align256(sizeof(LanczosState) * count);
+  bp.off_desc = off;  // everything below is uploaded in one copy
+  size_t doff = 0;
+  bp.d_lz = doff, doff += align256(sizeof(LanczosJob) * count);
+  bp.d_pad = doff, doff += align256(sizeof(PadJob) * count);
+  bp.d_split = doff, doff += align256(sizeof(SplitJob) * count);
+  bp.d_prod = doff, doff += align256(sizeof(ProdDesc) * (size_t)bp.nstages * count);
+  bp.d_tiles = doff, doff += align256(sizeof(TileRef) * tiles.size());
+  bp.total = off + doff;
+  if (!base) return PSDP_OK;
+
+  LanczosState* states = (LanczosState*)(base + bp.off_states);
+  bp.blob.assign(doff, 0);
+  char* blob = bp.blob.data();
+  LanczosJob* lz = (LanczosJob*)(blob + bp.d_lz);
+  PadJob* pj = (PadJob*)(blob + bp.d_pad);
+  SplitJob* sj = (SplitJob*)(blob + bp.d_split);
+  ProdDesc* pd = (ProdDesc*)(blob + bp.d_prod);
+  memcpy(blob + bp.d_tiles, tiles.data(), sizeof(TileRef) * tiles.size());
+
+  auto F = [&](size_t o) { return o != kNone ? (float*)(base + o) : (float*)nullptr; };
+  auto U = [&](size_t o) { return o != kNone ? (uint16_t*)(base + o) : (uint16_t*)nullptr; };
+  for (int b = 0; b < count; ++b) {
+    const BlockBufs& q = bb[b];
+    Bufs B;
+    B.a32 = F(q.a32);
+    B.p3 = F(q.p3);
+    B.p2 = F(q.p2);
+    B.x32 = F(q.x32);
+    for (int i = 0; i < 3; ++i) {
+      B.ap[i] = U(q.ap[i]);
+      B.p2p[i] = U(q.p2p[i]);
+      B.p3p[i] = U(q.p3p[i]);
+      B.xp[i] = U(q.xp[i]);
+    }
+    const size_t vb = align256((size_t)q.npad * 8);
+    char* v = base + q.vec;
+    LanczosJob& J = lz[b];
+    J.A = B.a32;
+    J.ld = q.npad;
+    J.n = q.n;
+    J.m = q.npad;
+    J.max_iter = q.iters;
+    J.st = states + b;
+    J.v.q = (double*)(v + 0 * vb);
+    J.v.w = (double*)(v + 1 * vb);
+    J.v.w1 = (double*)(v + 2 * vb);
+    J.v.vold = (double*)(v + 3 * vb);
+    J.v.y = (double*)(v + 4 * vb);
+    J.v.ay = (double*)(v + 5 * vb);
+    J.v.V = (double*)(v + 6 * vb);
+    pj[b] = PadJob{blocks[b].X, q.n, q.npad, B.a32};
+    SplitJob& S = sj[b];
+    S.a32 = B.a32;
+    S.count = (int64_t)q.npad * q.npad;
+    S.inv = &states[b].inv_scale;
+    S.x32 = B.x32;
+    for (int i = 0; i < 3; ++i) {
+      S.ap[i] = B.ap[i];
+      S.xp[i] = B.xp[i];
+    }
+    Plan p;
+    p.n = q.n;
+    p.npad = q.npad;
+    for (int i = 0; i < sch.nsteps; ++i) {
+      const int f = sch.fmt[i];
+      const int fn = i + 1 < sch.nsteps ? sch.fmt[i + 1] : sch.final_fmt;
+      const bool last = i + 1 == sch.nsteps;
+      ProdDesc& d1 = pd[(size_t)(3 * i) * count + b];
+      d1 = base_desc(p);
+      set_operand(d1.x, f, B.a32, B.ap);
+      set_operand(d1.y, f, B.a32, B.ap);
+      d1.mode = EPI_STORE;
+      d1.c32 = f == FMT_F32 ? B.p2 : nullptr;
+      d1.out_fmt = f;
+      for (int k = 0; k < 3; ++k) d1.outp[k] = B.p2p[k];
+      ProdDesc& d2 = pd[(size_t)(3 * i + 1) * count + b];
+      d2 = base_desc(p);
+      set_operand(d2.x, f, B.p2, B.p2p);
+      set_operand(d2.y, f, B.a32, B.ap);
+      d2.mode = EPI_STORE;
+      d2.c32 = B.p3;
+      d2.out_fmt = f;
+      for (int k = 0; k < 3; ++k) d2.outp[k] = B.p3p[k];
+      ProdDesc& d3 = pd[(size_t)(3 * i + 2) * count + b];
+      d3 = base_desc(p);
+      set_operand(d3.x, f, B.p3, B.p3p);
+      set_operand(d3.y, f, B.p2, B.p2p);
+      d3.mode = last ? EPI_POLY_HALF_ID : EPI_POLY;
+      d3.a = sch.a[i];
+      d3.b = sch.b[i];
+      d3.c = sch.c[i];
+      d3.c32 = B.a32;
+      d3.p3 = B.p3;
+      d3.out_fmt = fn;
+      for (int k = 0; k < 3; ++k) d3.outp[k] = B.ap[k];
+    }
+    ProdDesc& d4 = pd[(size_t)(3 * sch.nsteps) * count + b];
+    d4 = base_desc(p);
+    set_operand(d4.x, sch.final_fmt, B.x32, B.xp);
+    set_operand(d4.y, sch.final_fmt, B.a32, B.ap);
+    d4.mode = EPI_OUTPUT;
+    d4.out = blocks[b].Y;
+    d4.out_ld = q.n;
+    d4.out_f64 = 0;
+    d4.scale = &states[b].scale;
+  }
+  for (int st = 0; st < bp.nstages; ++st) {
+    const int f = st >= 3 * sch.nsteps ? sch.final_fmt : sch.fmt[st / 3];
+    for (int b = 0; b < count; ++b) DESCCHK(pd[(size_t)st * count + b], f);
+  }
+  for (int b = 0; b < count; ++b) {
+    if (!pj[b].X || !pj[b].M || !sj[b].a32 || !lz[b].A || !lz[b].v.q || !lz[b].v.V)
+      return fail(PSDP_ERR_INVALID_ARG, "internal: block %d has an unset buffer", b);
+    if (sch.fmt[0] != FMT_F32 && !sj[b].ap[0]) return fail(PSDP_ERR_INVALID_ARG, "internal: block %d: no A pieces", b);
+    if (sch.final_fmt == FMT_F32 ? !sj[b].x32 : !sj[b].xp[0])
+      return fail(PSDP_ERR_INVALID_ARG, "internal: block %d: no X_s buffer", b);
+  }
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_workspace_size_batched(const psdp_block_t* blocks, int count, const psdp_config_t* cfg,
+                                          size_t* bytes) {
+  if (!bytes) return fail(PSDP_ERR_INVALID_ARG, "null output");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  BatchPlan bp;
+  STCHK(plan_batched(blocks, count, cfg, nullptr, bp));
+  // Build and check every descriptor against a stand-in base address: the
+  // same host code as a real call, without touching a device.
+  BatchPlan chk;
+  STCHK(plan_batched(blocks, count, cfg, (char*)(uintptr_t)0x10000000, chk));
+  *bytes = bp.total;
   return PSDP_OK;
 }
 
 psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
                                        const psdp_config_t* cfg) {
-  (void)h;
-  (void)blocks;
-  (void)count;
-  (void)cfg;
-  return fail(PSDP_ERR_UNSUPPORTED, "batched projection not built yet");
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  BatchPlan bp;
+  STCHK(plan_batched(blocks, count, cfg, nullptr, bp));
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  void* base_v = nullptr;
+  STCHK(arena_reserve(h, bp.total, &base_v));
+  char* base = (char*)base_v;
+  STCHK(plan_batched(blocks, count, cfg, base, bp));
+  const Schedule& sch = bp.sch;
+  LanczosState* states = (LanczosState*)(base + bp.off_states);
+  char* ddev = base + bp.off_desc;
+
+  // initial states: scale 1 (used as is when auto_scale is off)
+  std::vector<LanczosState> st0(count);
+  memset(st0.data(), 0, sizeof(LanczosState) * count);
+  for (auto& x : st0) x.scale = x.inv_scale = 1.0, x.up = 1.0;
+  HIPCHK(hipMemcpyAsync(states, st0.data(), sizeof(LanczosState) * count, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ddev, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope below
+
+  const LanczosJob* lz_d = (const LanczosJob*)(ddev + bp.d_lz);
+  const ProdDesc* pd_d = (const ProdDesc*)(ddev + bp.d_prod);
+  const TileRef* tl_d = (const TileRef*)(ddev + bp.d_tiles);
+  const int grid = std::min(bp.ntiles, 1024);
+  {
+    ProfScope ps(h, PSDP_PROF_PREP);
+    HIPCHK(launch_sym_pad_batched((const PadJob*)(ddev + bp.d_pad), count, bp.max_npad, s));
+  }
+  if (cfg->auto_scale) {
+    ProfScope ps(h, PSDP_PROF_LANCZOS);
+    HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin, s));
+  }
+  {
+    ProfScope ps(h, PSDP_PROF_PREP);
+    HIPCHK(launch_scale_split_batched((const SplitJob*)(ddev + bp.d_split), count, bp.max_N, sch.fmt[0],
+                                      sch.final_fmt, s));
+  }
+  for (int st = 0; st < bp.nstages; ++st) {
+    const bool fin = st >= 3 * sch.nsteps;
+    const int f = fin ? sch.final_fmt : sch.fmt[st / 3];
+    const int avg = fin ? sch.final_avg : (st % 3 != 0 ? sch.avg[st / 3] : 0);
+    ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
+    HIPCHK(launch_sym_product_grouped(f, bp.sub, avg, pd_d + (size_t)st * count, tl_d, bp.ntiles, grid, s));
+  }
+  if (cfg->synchronize) {
+    std::vector<LanczosState> out(count);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(out.data(), states, sizeof(LanczosState) * count, hipMemcpyDeviceToHost));
+    for (int b = 0; b < count; ++b)
+      if (out[b].nonfinite) return fail(PSDP_ERR_NONFINITE, "block %d: non-finite value in the input", b);
+  }
+  return PSDP_OK;
+}
+
+static psdp_status_t eig_dropin(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues, int f64) {
+  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size %lld", (long long)n);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->blas) {
+    h->blas = eig_create_blas();
+    if (!h->blas) return fail(PSDP_ERR_DEVICE, "rocBLAS handle creation failed");
+  }
+  const size_t wsb = align256(eig_workspace_bytes((int)n, f64));
+  void* base = nullptr;
+  STCHK(arena_reserve(h, wsb + 256, &base));
+  int* info = (int*)((char*)base + wsb);
+  const int st = eig_project(h->blas, h->stream, mat, (int)n, f64, base, info, eigenvalues);
+  if (st) return fail(PSDP_ERR_DEVICE, "rocSOLVER/rocBLAS call failed (status %d)", st);
+  int info_h = 0;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&info_h, info, sizeof info_h, hipMemcpyDeviceToHost));
+  if (info_h != 0) return fail(PSDP_ERR_NONFINITE, "syevd did not converge (info = %d)", info_h);
+  return PSDP_OK;
 }
 
 psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
-  (void)h;
-  (void)mat;
-  (void)n;
-  (void)eigenvalues;
-  return fail(PSDP_ERR_UNSUPPORTED, "eig_fp64_psd not built yet");
+  return eig_dropin(h, mat, n, eigenvalues, 1);
 }
 
 psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
-  (void)h;
-  (void)mat;
-  (void)n;
-  (void)eigenvalues;
-  return fail(PSDP_ERR_UNSUPPORTED, "eig_fp32_psd not built yet");
+  return eig_dropin(h, mat, n, eigenvalues, 0);
 }
 
 }  // extern "C"

@@ -84,4 +84,34 @@ struct LanczosVecs {
   double* ay;
 };
 
+// One matrix of a batched Lanczos launch (float32 padded input).
+struct LanczosJob {
+  const float* A;
+  int64_t ld;
+  int n;         // logical size
+  int m;         // padded size (vector length)
+  int max_iter;  // min(iterations, n)
+  int pad;
+  LanczosState* st;
+  LanczosVecs v;
+};
+
+// One matrix of a batched symmetrize/pad launch.
+struct PadJob {
+  const float* X;  // column-major n x n, leading dimension n
+  int n;
+  int npad;
+  float* M;        // npad x npad
+};
+
+// One matrix of a batched scale/split launch.
+struct SplitJob {
+  float* a32;
+  int64_t count;
+  const double* inv;
+  uint16_t* ap[3];
+  float* x32;
+  uint16_t* xp[3];
+};
+
 }  // namespace psdp

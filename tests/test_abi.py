@@ -50,8 +50,8 @@ def test_default_config_and_workspace():
     N = 16384 * 16384
     # FP32 chain: A, A^3, A^2, X_s in float32 (+ Lanczos vectors)
     assert 16 * N <= ws_fp32 < 16 * N + (8 << 20)
-    # mixed: A, A^3 float32 + 2 pieces for each of A, A^2, A^3 and 1 for X_s
-    assert 22 * N <= ws_mixed < 22 * N + (8 << 20)
+    # mixed: A, A^3 float32 + 2 binary16 pieces for each of A, A^2, A^3, X_s
+    assert 24 * N <= ws_mixed < 24 * N + (8 << 20)
 
 
 def test_invalid_arguments_rejected_on_host():
@@ -73,3 +73,22 @@ def test_create_without_gpu_fails_cleanly():
         pytest.skip("GPU present")
     with pytest.raises(P.PsdpError):
         P.Handle(0)
+
+
+@pytest.mark.parametrize("method", ["mixed", "composite_FP32", "composite_FP16", "composite_FP32_emulated"])
+@pytest.mark.parametrize("sizes", [[1], [33], [32, 1024, 77, 300], list(range(1, 40, 3))])
+def test_batched_plan_host_checks(method, sizes):
+    # Builds every descriptor of a batched call on the host (stand-in device
+    # pointers) and checks that no operand or output buffer is missing; the
+    # workspace covers at least each block's own buffers.
+    b = P.workspace_size_batched(sizes, method)
+    single = sum(P.workspace_size(-(-n // 64) * 64 if n > 1024 else n, method) for n in sizes)
+    assert b > 0
+    assert b >= 0.5 * single
+
+
+def test_batched_plan_rejects_bad_input():
+    with pytest.raises(P.PsdpError):
+        P.workspace_size_batched([], "mixed")
+    with pytest.raises(P.PsdpError):
+        P.workspace_size_batched([10, 0], "mixed")

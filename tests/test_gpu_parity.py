@@ -53,19 +53,30 @@ def start_vector(seed, n):
 
 
 # ------------------------------------------------------------------ products
-@pytest.mark.parametrize("fmt", ["f32", "f16", "f16x3", "bf16", "bf16x3", "bf16x9"])
+@pytest.mark.parametrize("fmt,avg", [("f32", 0), ("f16", 0), ("f16x3", 0), ("bf16", 0), ("bf16x3", 0),
+                                     ("bf16x9", 0), ("f16", 1), ("bf16", 1)])
 @pytest.mark.parametrize("n", [64, 200, 1100])
-def test_sym_product(handle, fmt, n):
-    print(f"sym_product fmt={fmt} n={n}", flush=True)
+def test_sym_product(handle, fmt, avg, n):
+    # The kernel computes tiles with bi <= bj only; its result is the upper
+    # triangle of X Y mirrored (or, averaged, (X Y + Y X)/2).  Compared with
+    # the oracle's rounding-emulated product symmetrized the same way.
+    print(f"sym_product fmt={fmt} avg={avg} n={n}", flush=True)
     X, _, _ = O.spectral_input(n, 100 + n, "uniform")
     X32 = X.astype(np.float32)
     Y32 = (X @ X).astype(np.float32)
     Y32 = ((Y32 + Y32.T) / 2).astype(np.float32)
-    C = from_dev(P.sym_product(handle, to_dev(X32, torch.float32), to_dev(Y32, torch.float32), fmt))
-    ref = O.emulated_product(X32, Y32, fmt).astype(np.float64)
+    C = from_dev(P.sym_product(handle, to_dev(X32, torch.float32), to_dev(Y32, torch.float32), fmt,
+                               average=bool(avg)))
+    ref = O.sym_product(X32, Y32, fmt, "average" if avg else "mirror").astype(np.float64)
     assert np.array_equal(C, C.T)
     err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
     assert err <= 2e-6, err
+
+
+def test_sym_product_average_rejected_for_split_formats(handle):
+    X = torch.eye(8, device=DEV)
+    with pytest.raises(P.PsdpError):
+        P.sym_product(handle, X, X, "f16x3", average=True)
 
 
 # ------------------------------------------------------------------ KATs
@@ -126,18 +137,21 @@ def run_projection(handle, X, method, dtype=torch.float64, **kw):
 
 
 def oracle_error(X, P_true, method, seed):
+    """The oracle's error for the same method, with the library's
+    symmetrization: upper triangle mirrored, except the FP16 method, which
+    averages its non-square products like the reference's symmetrize."""
     n = X.shape[0]
     _, up = O.lanczos_two_norm(X, 20, 1e-10, q0=start_vector(seed, n))
     Xs = X / up
+    c32, c16 = O.fp32_chain_coefficients(), O.fp16_chain_coefficients()
     if method == "composite_FP32":
-        Y = O.composite_FP32(Xs)
+        Y = O.composite_chain(Xs, c32, ["f32"] * 10, "f32", sym_mode="mirror")
     elif method == "composite_FP16":
-        Y = O.composite_FP16(Xs)
+        Y = O.composite_chain(Xs, c16, ["f16"] * 7, "f16", sym_mode="average")
     elif method == "composite_FP32_emulated":
-        Y = O.composite_FP32_emulated(Xs)
+        Y = O.composite_chain(Xs, c32, ["bf16x9"] * 10, "bf16x9", sym_mode="mirror")
     elif method == "mixed":
-        c = O.fp16_chain_coefficients()
-        Y = O.composite_chain(Xs, c, ["f16x3"] * 3 + ["f16"] * 4, "f16")
+        Y = O.composite_chain(Xs, c16, ["f16x3"] * 4 + ["f16"] * 3, "f16x3", sym_mode="mirror")
     return O.rel_fro_error(Y * up, P_true)
 
 
@@ -148,7 +162,8 @@ def test_reference_uniform_scaled_1000(handle):
     P.composite_FP32(handle, d)
     Y = from_dev(d)
     err = O.rel_fro_error(Y, Ppsd)
-    e_or = O.rel_fro_error(O.composite_FP32(A), Ppsd)
+    e_or = O.rel_fro_error(O.composite_chain(A, O.fp32_chain_coefficients(), ["f32"] * 10, "f32",
+                                             sym_mode="mirror"), Ppsd)
     print("uniform_scaled rel err gpu", err, "oracle", e_or)
     assert err <= 1e-3
     assert err <= 1.5 * e_or + 2e-6
@@ -176,6 +191,7 @@ def test_reference_uniform_autoscaled_1024(handle, fn, method):
     ("composite_FP32", "uniform", 512, 2e-5),
     ("composite_FP32_emulated", "uniform", 512, 2e-5),
     ("composite_FP16", "cfg2", 512, None),
+    ("composite_FP16", "uniform", 512, None),
     ("mixed", "cfg2", 1024, 1e-3),
     ("mixed", "cfg3", 1024, 1e-3),
     ("mixed", "uniform", 640, 1e-3),
@@ -262,3 +278,64 @@ def test_explicit_workspace(handle):
     assert torch.equal(Y1, Y2)
     with pytest.raises(P.PsdpError):
         P.project(handle, to_dev(X), method="composite_FP32", workspace=ws[: need // 2])
+
+
+# ------------------------------------------------------------------ batched
+def batched_inputs(count, seed, lo=32, hi=300):
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    out = []
+    for i, n in enumerate(ns):
+        X, Q, lam = O.spectral_input(int(n), seed * 1000 + i, "uniform")
+        out.append((X, O.closed_form_projection(Q, lam)))
+    return out
+
+
+@pytest.mark.parametrize("method,bar", [("mixed", 1e-3), ("composite_FP32", 2e-5)])
+def test_batched_accuracy_and_batch_invariance(handle, method, bar):
+    data = batched_inputs(24, 4)
+    Xs = [to_dev(X, torch.float32) for X, _ in data]
+    handle.set_seed(11)
+    Ys = P.project_batched(handle, Xs, method=method)
+    worst = 0.0
+    for (X, Pt), Y in zip(data, Ys):
+        err = O.rel_fro_error(from_dev(Y), Pt)
+        worst = max(worst, err)
+        assert err <= bar, (X.shape[0], err)
+    print(f"batched {method}: worst block rel err {worst:.3e}")
+    # the same block alone (single-matrix path) gives bitwise the same result
+    for k in (0, 7, 23):
+        handle.set_seed(11)
+        alone = P.project(handle, Xs[k], method=method)
+        assert torch.equal(alone, Ys[k]), k
+    # and in a different mix (other blocks, other order)
+    handle.set_seed(11)
+    mix = [Xs[23]] + [to_dev(X, torch.float32) for X, _ in batched_inputs(5, 9)] + [Xs[0]]
+    Ym = P.project_batched(handle, mix, method=method)
+    assert torch.equal(Ym[0], Ys[23]) and torch.equal(Ym[-1], Ys[0])
+
+
+def test_batched_rejects_bad_blocks(handle):
+    with pytest.raises(P.PsdpError):
+        P.project_batched(handle, [])
+
+
+# ------------------------------------------------------------------ EVD drop-ins
+def test_eig_fp64_psd(handle):
+    X, Q, lam = O.spectral_input(300, 21, "normal")
+    d = to_dev(X)
+    w = P.eig_FP64_psd(handle, d, return_eigenvalues=True)
+    assert O.rel_fro_error(from_dev(d), O.closed_form_projection(Q, lam)) < 1e-12
+    assert np.allclose(w.cpu().numpy(), np.sort(lam), atol=1e-12)
+    k = KATS["simple5"]
+    d5 = to_dev(np.diag(k["diagonal"]))
+    assert P.eig_FP64_psd(handle, d5) is None
+    assert np.abs(from_dev(d5) - np.diag(k["expected_diagonal"])).max() < 1e-14
+
+
+def test_eig_fp32_psd(handle):
+    X, Q, lam = O.spectral_input(200, 22, "normal")
+    d = to_dev(X)
+    w = P.eig_FP32_psd(handle, d, return_eigenvalues=True)
+    assert O.rel_fro_error(from_dev(d), O.closed_form_projection(Q, lam)) < 2e-5
+    assert np.allclose(w.cpu().numpy(), np.sort(lam), atol=1e-4)

@@ -257,3 +257,51 @@ def test_kat_lobpcg_eigenvalues():
     k = KATS["lobpcg_6x6"]
     ev = sorted(k["eigenvalues"], reverse=True)[:2]
     assert np.allclose(ev, k["expected_top2_reference"], atol=k["tol"])
+
+
+# ----------------------------------------------------------------- symmetrization
+def test_average_mode_tracks_reference_symmetrize():
+    # Averaging every product with its transpose keeps the reference's error
+    # (which symmetrizes A once per step by averaging); mirroring the upper
+    # triangle of plain-FP16 products roughly doubles it.
+    X, Q, lam = O.spectral_input(384, 2, "cfg2")
+    Pt = O.closed_form_projection(Q, lam)
+    _, up = O.lanczos_two_norm(X)
+    Xs = (X / up).astype(np.float32)
+    c = O.fp16_chain_coefficients()
+    err = {m: O.rel_fro_error(O.composite_chain(Xs, c, ["f16"] * 7, "f16", sym_mode=m) * up, Pt)
+           for m in ("reference", "average", "mirror")}
+    assert err["average"] == pytest.approx(err["reference"], rel=0.15)
+    assert err["mirror"] > 1.5 * err["reference"]
+
+
+def test_mirror_and_average_products():
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((40, 40))
+    X = ((G + G.T) / 8).astype(np.float32)
+    Y = (X.astype(np.float64) @ X).astype(np.float32)
+    C = O.emulated_product(X, Y, "f16")
+    M = O.sym_product(X, Y, "f16", "mirror")
+    A = O.sym_product(X, Y, "f16", "average")
+    iu = np.triu_indices(40)
+    assert np.array_equal(M[iu], C[iu]) and np.array_equal(M, M.T)
+    assert np.allclose(A, (C + C.T) / 2) and np.array_equal(A, A.T)
+
+
+@pytest.mark.parametrize("a", [-3.0, 2.0, 0.25, 1e-3])
+def test_lanczos_one_by_one(a):
+    # 1x1: q is an eigenvector, beta_0 = 0; the exact Ritz value a^2 is kept
+    # and up = sqrt(a^2 + |a - a^2|) >= |a|.  The reference's own rule would
+    # report (0, 1) here (it drops the iteration), which the chain then
+    # applies to an unscaled |a| > 1.
+    lo, up = O.lanczos_two_norm(np.array([[a]]), 20, 1e-10, q0=np.ones(1))
+    assert lo == pytest.approx(abs(a), rel=1e-15)
+    assert up == pytest.approx(np.sqrt(a * a + abs(a - a * a)), rel=1e-15)
+    assert O.lanczos_two_norm(np.array([[a]]), 20, 1e-10, q0=np.ones(1), keep_exact=False) == (0.0, 1.0)
+
+
+def test_lanczos_invariant_start_vector():
+    # q0 = e_0 spans an eigenspace of diag(d): beta_0 = 0 at once
+    d = np.array([-4.0, 1.0, 2.0, 3.0])
+    lo, up = O.lanczos_two_norm(np.diag(d), 20, 1e-10, q0=np.eye(4)[0])
+    assert lo == pytest.approx(4.0) and up == pytest.approx(np.sqrt(16 + 20))
