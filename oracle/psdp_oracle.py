@@ -126,7 +126,7 @@ def round_bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-F16X3_LO_SCALE = 2048.0  # 2**11: keeps the low FP16 part out of the subnormal range
+F16X3_SCALE = 256.0  # 2**8: s*x split into hi + lo binary16 keeps lo out of the subnormal range
 
 
 def split_operand(x, fmt):
@@ -139,9 +139,9 @@ def split_operand(x, fmt):
     if fmt == "f16":
         return [round_f16(x)]
     if fmt == "f16x3":
-        hi = round_f16(x)
-        lo = round_f16((x - hi) * np.float32(F16X3_LO_SCALE))
-        return [hi, lo]
+        w = x * np.float32(F16X3_SCALE)
+        hi = round_f16(w)
+        return [hi, round_f16(w - hi)]
     if fmt == "bf16":
         return [round_bf16(x)]
     if fmt == "bf16x3":
@@ -165,8 +165,8 @@ def emulated_product(x, y, fmt):
     if fmt in ("f32", "f16", "bf16"):
         return mm(px[0], py[0])
     if fmt == "f16x3":
-        cross = mm(px[0], py[1]) + mm(px[1], py[0])
-        return mm(px[0], py[0]) + cross * np.float32(1.0 / F16X3_LO_SCALE)
+        acc = mm(px[1], py[0]) + mm(px[0], py[1])
+        return (acc + mm(px[0], py[0])) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
     if fmt == "bf16x3":
         return mm(px[0], py[0]) + (mm(px[0], py[1]) + mm(px[1], py[0]))
     if fmt == "bf16x6":

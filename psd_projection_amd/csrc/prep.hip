@@ -11,9 +11,24 @@
 namespace psdp {
 
 // M = (X + X^T)/2 as float32, zero padded to npad x npad (row-major view,
-// leading dimension ld).  X is column-major n x n with leading dimension xld.
+// leading dimension ld >= npad; columns npad .. ld-1 are zeroed too, by the
+// workgroups of the last column block, so that flat passes over npad * ld
+// values see only finite data).  X is column-major n x n with leading
+// dimension xld.
 // One 64x64 tile pair (bi <= bj) per workgroup: both blocks are staged in
 // LDS so that the transposed block is read and written coalesced.
+// Columns npad .. ld-1 of the 64 rows of block row bi, written by the
+// workgroup of the last column block (npad = 64 * gridDim.x).
+__device__ __forceinline__ void zero_row_tail(float* __restrict__ M, int64_t ld, int bi, int bj) {
+  const int64_t npad = 64 * (int64_t)gridDim.x;
+  if (bj != (int)gridDim.x - 1 || ld == npad) return;
+  const int extra = (int)(ld - npad);
+  for (int e = threadIdx.x; e < 64 * extra; e += blockDim.x) {
+    const int a = e / extra, b = e % extra;
+    M[(int64_t)(bi * 64 + a) * ld + npad + b] = 0.0f;
+  }
+}
+
 template <typename TIn>
 __global__ __launch_bounds__(256) void sym_pad_kernel(const TIn* __restrict__ X, int64_t xld, int n,
                                                       float* __restrict__ M, int64_t ld) {
@@ -40,6 +55,7 @@ __global__ __launch_bounds__(256) void sym_pad_kernel(const TIn* __restrict__ X,
     M[(int64_t)(i0 + a) * ld + j0 + b] = 0.5f * (s1[b][a] + s2[a][b]);
     if (bi != bj) M[(int64_t)(j0 + a) * ld + i0 + b] = 0.5f * (s2[b][a] + s1[a][b]);
   }
+  zero_row_tail(M, ld, bi, bj);
 }
 
 // Symmetric inputs given in double are averaged in double before rounding.
@@ -68,6 +84,7 @@ __global__ __launch_bounds__(256) void sym_pad_kernel<double>(const double* __re
     M[(int64_t)(i0 + a) * ld + j0 + b] = (float)(0.5 * (s1[b][a] + s2[a][b]));
     if (bi != bj) M[(int64_t)(j0 + a) * ld + i0 + b] = (float)(0.5 * (s2[b][a] + s1[a][b]));
   }
+  zero_row_tail(M, ld, bi, bj);
 }
 
 struct SplitArgs {

@@ -27,6 +27,7 @@
 namespace psdp {
 hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, const TileRef* tiles, int ntiles,
                               hipStream_t s);
+bool sym_big_tile_capable(int fmt, int avg);
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,
                                       int ntiles, int grid, hipStream_t s);
 hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
@@ -333,8 +334,14 @@ static bool desc_ok(const ProdDesc& d, int fmt, const char** why) {
   } while (0)
 
 // ---------------------------------------------------------------- plan
+// Row stride of the workspace matrices: npad + LD_PAD elements.  A
+// power-of-two stride (npad = 16384 is 32 KB per binary16 row) maps every row
+// of a tile panel to the same cache sets and channels; 64 extra elements
+// (128 B for binary16, 256 B for float32) spread them.
+static constexpr int LD_PAD = 64;
+
 struct Plan {
-  int n = 0, npad = 0, sub = 4, bt = 128;
+  int n = 0, npad = 0, ld = 0, sub = 4, bt = 128;
   int64_t N = 0;
   Schedule sch;
   int lanczos_iters = 20;
@@ -349,6 +356,13 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int choose_sub(int64_t n) { return n > 1024 ? 4 : 2; }
 
+// Tile size of one product: 256 x 256 for large n where the kernel exists,
+// else 128 (n > 1024) or 64.
+static int launch_sub(int64_t n, int fmt, int avg) {
+  if (n >= 2048 && sym_big_tile_capable(fmt, avg)) return 8;
+  return choose_sub(n);
+}
+
 static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
   if (n <= 0) return fail(PSDP_ERR_INVALID_ARG, "matrix size must be positive (got %lld)", (long long)n);
   if (n > (1 << 20)) return fail(PSDP_ERR_INVALID_ARG, "matrix size %lld too large", (long long)n);
@@ -358,9 +372,15 @@ static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
     return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
   p.n = (int)n;
   p.sub = choose_sub(n);
-  p.bt = 32 * p.sub;
+  for (int i = 0; i < p.sch.nsteps; ++i) {
+    p.sub = std::max(p.sub, launch_sub(n, p.sch.fmt[i], 0));
+    p.sub = std::max(p.sub, launch_sub(n, p.sch.fmt[i], p.sch.avg[i]));
+  }
+  p.sub = std::max(p.sub, launch_sub(n, p.sch.final_fmt, p.sch.final_avg));
+  p.bt = 32 * p.sub;  // the largest tile used: npad is a multiple of every tile
   p.npad = (int)((n + p.bt - 1) / p.bt * p.bt);
-  p.N = (int64_t)p.npad * p.npad;
+  p.ld = p.npad + LD_PAD;
+  p.N = (int64_t)p.npad * p.ld;
   bool any_f32 = false;
   int pa = pieces_of(p.sch.final_fmt), pp = 0;
   for (int i = 0; i < p.sch.nsteps; ++i) {
@@ -427,7 +447,7 @@ static void set_operand(const void* (&dst)[3], int fmt, const float* m32, uint16
 static ProdDesc base_desc(const Plan& p) {
   ProdDesc d;
   memset(&d, 0, sizeof d);
-  d.ld = p.npad;
+  d.ld = p.ld;
   d.npad = p.npad;
   d.n = p.n;
   return d;
@@ -437,20 +457,28 @@ static ProdDesc base_desc(const Plan& p) {
 static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
                                         void* Y, int y_f64, const psdp_config_t* cfg) {
   hipStream_t s = h->stream;
-  TileList tl;
-  STCHK(get_tiles(h, p.npad / p.bt, p.bt, tl));
   const Schedule& sch = p.sch;
+  // one product: tile size by format, then the launch
+  auto product = [&](int fmt, int avg, const ProdDesc& d) -> psdp_status_t {
+    DESCCHK(d, fmt);
+    const int sub = launch_sub(p.n, fmt, avg);
+    TileList tl;
+    STCHK(get_tiles(h, p.npad / (32 * sub), 32 * sub, tl));
+    ProfScope ps(h, PSDP_PROF_PROD_F32 + fmt);
+    HIPCHK(launch_sym_product(fmt, sub, avg, d, tl.dev, tl.count, s));
+    return PSDP_OK;
+  };
 
   {
     ProfScope ps(h, PSDP_PROF_PREP);
-    HIPCHK(launch_sym_pad(X, x_f64, p.n, p.n, b.a32, p.npad, p.npad, s));
+    HIPCHK(launch_sym_pad(X, x_f64, p.n, p.n, b.a32, p.ld, p.npad, s));
   }
   if (cfg->auto_scale) {
     const double tol = cfg->lanczos_tol > 0 ? cfg->lanczos_tol : 1e-10;
     const double margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
     const int iters = std::min(p.lanczos_iters, p.n);  // max_iter = min(max_iter, n), src/lanczos.cu:55
     ProfScope ps(h, PSDP_PROF_LANCZOS);
-    HIPCHK(launch_lanczos(b.a32, 0, p.npad, p.n, p.npad, iters, tol, h->seed, margin, h->st, b.v, s));
+    HIPCHK(launch_lanczos(b.a32, 0, p.ld, p.n, p.npad, iters, tol, h->seed, margin, h->st, b.v, s));
   } else {
     HIPCHK(launch_set_scale(h->st, 1.0, s));
   }
@@ -471,11 +499,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d1.c32 = f == FMT_F32 ? b.p2 : nullptr;
     d1.out_fmt = f;
     for (int q = 0; q < 3; ++q) d1.outp[q] = b.p2p[q];
-    {
-      DESCCHK(d1, f);
-      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, 0, d1, tl.dev, tl.count, s));
-    }
+    STCHK(product(f, 0, d1));
     // A3 = A2 A
     ProdDesc d2 = base_desc(p);
     set_operand(d2.x, f, b.p2, b.p2p);
@@ -484,11 +508,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d2.c32 = b.p3;
     d2.out_fmt = f;
     for (int q = 0; q < 3; ++q) d2.outp[q] = b.p3p[q];
-    {
-      DESCCHK(d2, f);
-      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, sch.avg[i], d2, tl.dev, tl.count, s));
-    }
+    STCHK(product(f, sch.avg[i], d2));
     // A = a A + b A3 + c A3 A2 (last step: S = (I + A)/2)
     ProdDesc d3 = base_desc(p);
     set_operand(d3.x, f, b.p3, b.p3p);
@@ -501,11 +521,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d3.p3 = b.p3;
     d3.out_fmt = fn;
     for (int q = 0; q < 3; ++q) d3.outp[q] = b.ap[q];
-    {
-      DESCCHK(d3, f);
-      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-      HIPCHK(launch_sym_product(f, p.sub, sch.avg[i], d3, tl.dev, tl.count, s));
-    }
+    STCHK(product(f, sch.avg[i], d3));
   }
   // W = X_s S, Y = s W
   ProdDesc d4 = base_desc(p);
@@ -516,11 +532,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   d4.out_ld = p.n;
   d4.out_f64 = y_f64;
   d4.scale = &h->st->scale;
-  {
-    DESCCHK(d4, sch.final_fmt);
-    ProfScope ps(h, PSDP_PROF_PROD_F32 + sch.final_fmt);
-    HIPCHK(launch_sym_product(sch.final_fmt, p.sub, sch.final_avg, d4, tl.dev, tl.count, s));
-  }
+  STCHK(product(sch.final_fmt, sch.final_avg, d4));
   return PSDP_OK;
 }
 
@@ -799,9 +811,9 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
     return fail(PSDP_ERR_UNSUPPORTED, "averaged products exist for binary16 and bfloat16 operands only");
   std::lock_guard<std::mutex> lock(h->mu);
   HIPCHK(hipSetDevice(h->device));
-  const int sub = choose_sub(n), bt = 32 * sub;
-  const int npad = (int)((n + bt - 1) / bt * bt);
-  const int64_t N = (int64_t)npad * npad;
+  const int sub = launch_sub(n, format, average ? 1 : 0), bt = 32 * sub;
+  const int npad = (int)((n + bt - 1) / bt * bt), ld = npad + LD_PAD;
+  const int64_t N = (int64_t)npad * ld;
   const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
   void* base = nullptr;
   STCHK(arena_reserve(h, 2 * f32b + 6 * u16b, &base));
@@ -817,14 +829,17 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   hipStream_t s = h->stream;
   TileList tl;
   STCHK(get_tiles(h, npad / bt, bt, tl));
-  HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, npad, npad, s));
-  HIPCHK(launch_sym_pad(Y, 0, n, (int)n, my, npad, npad, s));
-  uint16_t* none[3] = {nullptr, nullptr, nullptr};
-  HIPCHK(launch_scale_split(mx, N, nullptr, format, px, FMT_F32, mx, none, s));
-  HIPCHK(launch_scale_split(my, N, nullptr, format, py, FMT_F32, my, none, s));
+  {
+    ProfScope ps(h, PSDP_PROF_PREP);
+    HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, ld, npad, s));
+    HIPCHK(launch_sym_pad(Y, 0, n, (int)n, my, ld, npad, s));
+    uint16_t* none[3] = {nullptr, nullptr, nullptr};
+    HIPCHK(launch_scale_split(mx, N, nullptr, format, px, FMT_F32, mx, none, s));
+    HIPCHK(launch_scale_split(my, N, nullptr, format, py, FMT_F32, my, none, s));
+  }
   ProdDesc d;
   memset(&d, 0, sizeof d);
-  d.ld = npad;
+  d.ld = ld;
   d.npad = npad;
   d.n = (int)n;
   set_operand(d.x, format, mx, px);
@@ -834,7 +849,10 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   d.out_ld = n;
   d.out_f64 = 0;
   d.scale = nullptr;
-  HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
+  {
+    ProfScope ps(h, PSDP_PROF_PROD_F32 + format);
+    HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
+  }
   HIPCHK(hipStreamSynchronize(s));
   return PSDP_OK;
 }
@@ -987,6 +1005,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     Plan p;
     p.n = q.n;
     p.npad = q.npad;
+    p.ld = q.npad;  // blocks are small: unpadded rows
     for (int i = 0; i < sch.nsteps; ++i) {
       const int f = sch.fmt[i];
       const int fn = i + 1 < sch.nsteps ? sch.fmt[i + 1] : sch.final_fmt;

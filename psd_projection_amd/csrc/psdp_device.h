@@ -33,15 +33,16 @@ __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
 }
 
 // Splits a float32 value into the pieces of operand format `fmt`.
-// F16X3: hi = f16(v), lo = f16((v - hi) * 2^11).  BF16X3: hi, v - hi.
+// F16X3: w = 2^8 v (exact), hi = f16(w), lo = f16(w - hi).  BF16X3: hi, v - hi.
 // BF16X9: hi, mid, lo (three bfloat16 terms).
 __device__ __forceinline__ void split_value(int fmt, float v, uint16_t out[3]) {
   switch (fmt) {
     case FMT_F16: out[0] = f32_to_f16_bits(v); break;
     case FMT_F16X3: {
-      uint16_t h = f32_to_f16_bits(v);
+      const float w = v * F16X3_SCALE;
+      uint16_t h = f32_to_f16_bits(w);
       out[0] = h;
-      out[1] = f32_to_f16_bits((v - f16_bits_to_f32(h)) * F16X3_LO_SCALE);
+      out[1] = f32_to_f16_bits(w - f16_bits_to_f32(h));
       break;
     }
     case FMT_BF16: out[0] = f32_to_bf16_bits(v); break;

@@ -8,15 +8,19 @@ namespace psdp {
 enum : int {
   FMT_F32 = 0,     // float32 operands on the FP32 MFMA
   FMT_F16 = 1,     // binary16 operands, float32 accumulation
-  FMT_F16X3 = 2,   // hi + 2^-11 lo binary16 split, 3 products
+  FMT_F16X3 = 2,   // hi + lo binary16 split of 2^8 v, 3 products, one accumulator
   FMT_BF16 = 3,    // bfloat16 operands
   FMT_BF16X3 = 4,  // hi + lo bfloat16 split, 3 products
   FMT_BF16X9 = 5,  // hi + mid + lo bfloat16 split, 9 products
   FMT_COUNT = 6
 };
 
-constexpr float F16X3_LO_SCALE = 2048.0f;
-constexpr float F16X3_LO_INV = 1.0f / 2048.0f;
+// FMT_F16X3 splits s*v (s = 2^8, exact) into hi + lo binary16 pieces so that
+// lo stays out of the binary16 subnormal range for the entry sizes of a
+// matrix with spectrum in [-1, 1]; products carry s^2 and are rescaled by
+// 2^-16 in the epilogue.  Operands must satisfy |v| < 2^8.
+constexpr float F16X3_SCALE = 256.0f;
+constexpr float F16X3_PROD_SCALE = 1.0f / 65536.0f;
 
 // Epilogues of the symmetric product kernel.
 enum : int {

@@ -12,12 +12,16 @@
 // C[i][j] = sum_k X[i][k] Y[j][k]: both operands are read along contiguous k,
 // 16 bytes per lane.  Only tiles with bi <= bj are computed; each result is
 // written at (i, j) and (j, i), so every output is exactly symmetric (this
-// replaces the reference's explicit symmetrize passes).
+// replaces the reference's explicit symmetrize passes).  The averaged form
+// (AVG) accumulates C and C^T on the same tile and stores (C + C^T)/2.
 //
 // Tiling: 256 threads = 2 x 2 waves; each wave owns a (16*SUB) x (16*SUB)
-// sub-block made of 16x16 MFMA tiles.  K is staged through LDS in BK-wide
-// slabs, double buffered (one barrier per slab).  Rows are padded by 16 bytes
-// so that the 16-byte fragment reads of 16 consecutive rows hit distinct banks.
+// sub-block made of 16x16 MFMA tiles (SUB = 8 gives 256 x 256 block tiles
+// with 128 x 128 per wave, the shape used for large n; SUB = 4 and 2 give
+// 128 and 64).  K is staged through LDS in BK-wide slabs, double buffered
+// (one barrier per slab).  Rows are padded by 16 bytes so that the 16-byte
+// fragment reads of 16 consecutive rows hit distinct banks.  Split formats
+// (x3, x9) accumulate every partial product into one accumulator.
 #include "psdp_device.h"
 
 namespace psdp {
@@ -27,45 +31,44 @@ struct FmtTraits;
 template <>
 struct FmtTraits<FMT_F32> {
   using E = float;
-  static constexpr int P = 1, NACC = 1, BK = 32;
+  static constexpr int P = 1, BK = 32;
 };
 template <>
 struct FmtTraits<FMT_F16> {
   using E = uint16_t;
-  static constexpr int P = 1, NACC = 1, BK = 64;
+  static constexpr int P = 1, BK = 64;
 };
 template <>
 struct FmtTraits<FMT_F16X3> {
   using E = uint16_t;
-  static constexpr int P = 2, NACC = 2, BK = 32;
+  static constexpr int P = 2, BK = 32;
 };
 template <>
 struct FmtTraits<FMT_BF16> {
   using E = uint16_t;
-  static constexpr int P = 1, NACC = 1, BK = 64;
+  static constexpr int P = 1, BK = 64;
 };
 template <>
 struct FmtTraits<FMT_BF16X3> {
   using E = uint16_t;
-  static constexpr int P = 2, NACC = 1, BK = 32;
+  static constexpr int P = 2, BK = 32;
 };
 template <>
 struct FmtTraits<FMT_BF16X9> {
   using E = uint16_t;
-  static constexpr int P = 3, NACC = 1, BK = 32;
+  static constexpr int P = 3, BK = 32;
 };
 
 template <int FMT, int SUB, bool AVG = false>
 struct Tiling {
   using F = FmtTraits<FMT>;
-  static constexpr int NPROD = AVG ? 2 : 1;    // averaged products keep C and C^T
   using E = typename F::E;
+  static constexpr int NPROD = AVG ? 2 : 1;    // averaged products keep C and C^T
   static constexpr int ESZ = (int)sizeof(E);
   static constexpr int WT = 16 * SUB;          // wave tile
   static constexpr int BT = 2 * WT;            // block tile
   static constexpr int BK = F::BK;
   static constexpr int P = F::P;
-  static constexpr int NACC = F::NACC;
   static constexpr int VEC = 16 / ESZ;         // elements per 16-byte chunk
   static constexpr int ROW = BK + VEC;         // padded LDS row (elements)
   static constexpr int TILE = BT * ROW;        // elements of one piece tile
@@ -74,10 +77,12 @@ struct Tiling {
   static constexpr int LOADS = CHUNKS / 256;   // chunks per thread per piece tile
   static constexpr int STAGE = 2 * NPROD * P * TILE;  // X_I, Y_J (then Y_I, X_J)
   static constexpr int LDS_MAIN = 2 * STAGE * ESZ;
-  static constexpr int LDS_CTILE = BT * (BT + 1) * 4;
+  static constexpr int HR = BT < 128 ? BT : 128;      // epilogue rows staged at once
+  static constexpr int LDS_CTILE = HR * (BT + 1) * 4;
   static constexpr int LDS_BYTES = LDS_MAIN > LDS_CTILE ? LDS_MAIN : LDS_CTILE;
   static constexpr int SMEM_ELEMS = (LDS_BYTES + ESZ - 1) / ESZ;
   static_assert(CHUNKS % 256 == 0, "tile too small for 256 threads");
+  static_assert(LDS_BYTES <= 163840, "LDS budget of one workgroup exceeded");
 };
 
 __device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c, int fmt) {
@@ -113,37 +118,40 @@ __device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int
   if (np > 2) outp[2][idx] = p[2];
 }
 
-// Epilogue from a BT x BT float tile C staged in LDS (row stride BT + 1).
-// Two passes: the primary block (i, j) is written along j, the mirror block
-// (j, i) along i, so both are coalesced 16-byte stores.  Diagonal tiles write
-// only i <= j in the first pass and i < j in the second.
+// Epilogue for tile rows [r0, r0 + NR) staged in LDS (C holds local rows
+// 0..NR-1, row stride BT + 1).  The primary block (i, j) is written along j,
+// the mirror block (j, i) along i, so both are coalesced 16-byte stores.
+// Tiles on the diagonal write only i <= j first and i < j in the mirror; a
+// mirror store never lands on an element another phase reads (it is below
+// the diagonal), so the phases of one tile do not interfere.
 template <int BT>
-__device__ __forceinline__ void epilogue_from_lds(const ProdDesc& d, int bi, int bj, float* C) {
+__device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR) {
   constexpr int CS = BT + 1;
   constexpr int Q4 = BT / 4;
   const int tid = threadIdx.x;
-  const int row0 = bi * BT, col0 = bj * BT;
+  const int row0 = bi * BT + r0, col0 = bj * BT;
   const bool diag = bi == bj;
   const int64_t ld = d.ld;
+  const int nq = NR / 4;
   if (d.mode == EPI_OUTPUT) {
     const double s = d.scale ? *d.scale : 1.0;
     const int64_t old = d.out_ld;
     // (i, j) at i + j*ld: contiguous along i
-    for (int q = tid; q < BT * Q4; q += 256) {
-      const int c = q / Q4, r0 = (q % Q4) * 4;
+    for (int q = tid; q < BT * nq; q += 256) {
+      const int c = q / nq, rr = (q % nq) * 4;
       const int j = col0 + c;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = row0 + r0 + e;
+        const int i = row0 + rr + e;
         if (i < d.n && j < d.n && (!diag || i <= j)) {
-          const double v = (double)C[(r0 + e) * CS + c] * s;
+          const double v = (double)C[(rr + e) * CS + c] * s;
           if (d.out_f64) ((double*)d.out)[(int64_t)i + (int64_t)j * old] = v;
           else ((float*)d.out)[(int64_t)i + (int64_t)j * old] = (float)v;
         }
       }
     }
     // (j, i) at j + i*ld: contiguous along j
-    for (int q = tid; q < BT * Q4; q += 256) {
+    for (int q = tid; q < NR * Q4; q += 256) {
       const int r = q / Q4, c0 = (q % Q4) * 4;
       const int i = row0 + r;
 #pragma unroll
@@ -162,7 +170,7 @@ __device__ __forceinline__ void epilogue_from_lds(const ProdDesc& d, int bi, int
   const bool half_id = d.mode == EPI_POLY_HALF_ID;
   const bool pieces = d.out_fmt != FMT_F32;
   // pass 1: primary block, rows of the tile
-  for (int q = tid; q < BT * Q4; q += 256) {
+  for (int q = tid; q < NR * Q4; q += 256) {
     const int r = q / Q4, c0 = (q % Q4) * 4;
     const int i = row0 + r, j0 = col0 + c0;
     const int64_t idx = (int64_t)i * ld + j0;
@@ -200,13 +208,13 @@ __device__ __forceinline__ void epilogue_from_lds(const ProdDesc& d, int bi, int
   }
   __syncthreads();
   // pass 2: mirror block, columns of the tile
-  for (int q = tid; q < BT * Q4; q += 256) {
-    const int c = q / Q4, r0 = (q % Q4) * 4;
-    const int j = col0 + c, i0 = row0 + r0;
+  for (int q = tid; q < BT * nq; q += 256) {
+    const int c = q / nq, rr = (q % nq) * 4;
+    const int j = col0 + c, i0 = row0 + rr;
     const int64_t idx = (int64_t)j * ld + i0;
     float w[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) w[e] = C[(r0 + e) * CS + c];
+    for (int e = 0; e < 4; ++e) w[e] = C[(rr + e) * CS + c];
     if (!diag) {
       if (d.c32) *(f32x4*)(d.c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
       if (pieces) store_pieces4(d.out_fmt, d.outp, idx, w);
@@ -222,11 +230,11 @@ __device__ __forceinline__ void epilogue_from_lds(const ProdDesc& d, int bi, int
   }
 }
 
-// MFMAs over one BK-wide LDS slab (X pieces then Y pieces).
+// MFMAs over one BK-wide LDS slab: acc += X-rows x Y-rows^T.
 template <int FMT, int SUB>
 __device__ __forceinline__ void mma_slab(const typename Tiling<FMT, SUB>::E* X,
                                          const typename Tiling<FMT, SUB>::E* Y, int arow, int brow, int lane,
-                                         f32x4 (&acc)[Tiling<FMT, SUB>::NACC][SUB][SUB]) {
+                                         f32x4 (&acc)[SUB][SUB]) {
   using TL = Tiling<FMT, SUB>;
   constexpr int P = TL::P, BK = TL::BK, ROW = TL::ROW, TILE = TL::TILE;
   if constexpr (FMT == FMT_F32) {
@@ -245,46 +253,43 @@ __device__ __forceinline__ void mma_slab(const typename Tiling<FMT, SUB>::E* X,
         for (int si = 0; si < SUB; ++si)
 #pragma unroll
           for (int sj = 0; sj < SUB; ++sj)
-            acc[0][si][sj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[si][q], bf[sj][q], acc[0][si][sj], 0, 0, 0);
+            acc[si][sj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[si][q], bf[sj][q], acc[si][sj], 0, 0, 0);
     }
   } else {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 32) {
-      u32x4 af[P][SUB], bf[P][SUB];
+      // A fragments of all SUB row blocks stay live; B fragments are read one
+      // column block at a time (same LDS reads, fewer live registers).
+      u32x4 af[P][SUB];
       const int ko = kk + (lane >> 4) * 8;
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int s = 0; s < SUB; ++s) {
-          af[p][s] = *(const u32x4*)(X + p * TILE + arow + s * 16 * ROW + ko);
-          bf[p][s] = *(const u32x4*)(Y + p * TILE + brow + s * 16 * ROW + ko);
-        }
+        for (int s = 0; s < SUB; ++s) af[p][s] = *(const u32x4*)(X + p * TILE + arow + s * 16 * ROW + ko);
 #pragma unroll
-      for (int si = 0; si < SUB; ++si)
+      for (int sj = 0; sj < SUB; ++sj) {
+        u32x4 bf[P];
 #pragma unroll
-        for (int sj = 0; sj < SUB; ++sj) {
-          if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
-            acc[0][si][sj] = mfma16(af[0][si], bf[0][sj], acc[0][si][sj], FMT);
-          } else if constexpr (FMT == FMT_F16X3) {
-            acc[1][si][sj] = mfma16(af[0][si], bf[1][sj], acc[1][si][sj], FMT);
-            acc[1][si][sj] = mfma16(af[1][si], bf[0][sj], acc[1][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[0][si], bf[0][sj], acc[0][si][sj], FMT);
-          } else if constexpr (FMT == FMT_BF16X3) {
-            acc[0][si][sj] = mfma16(af[0][si], bf[1][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[1][si], bf[0][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[0][si], bf[0][sj], acc[0][si][sj], FMT);
-          } else {  // BF16X9: smallest terms first
-            acc[0][si][sj] = mfma16(af[2][si], bf[2][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[1][si], bf[2][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[2][si], bf[1][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[1][si], bf[1][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[0][si], bf[2][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[2][si], bf[0][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[0][si], bf[1][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[1][si], bf[0][sj], acc[0][si][sj], FMT);
-            acc[0][si][sj] = mfma16(af[0][si], bf[0][sj], acc[0][si][sj], FMT);
+        for (int p = 0; p < P; ++p) bf[p] = *(const u32x4*)(Y + p * TILE + brow + sj * 16 * ROW + ko);
+#pragma unroll
+        for (int si = 0; si < SUB; ++si) {
+          f32x4 c = acc[si][sj];
+          if constexpr (P == 2) {  // x3: lo*hi, hi*lo, then hi*hi
+            c = mfma16(af[1][si], bf[0], c, FMT);
+            c = mfma16(af[0][si], bf[1], c, FMT);
+          } else if constexpr (P == 3) {  // x9: smallest terms first
+            c = mfma16(af[2][si], bf[2], c, FMT);
+            c = mfma16(af[1][si], bf[2], c, FMT);
+            c = mfma16(af[2][si], bf[1], c, FMT);
+            c = mfma16(af[1][si], bf[1], c, FMT);
+            c = mfma16(af[0][si], bf[2], c, FMT);
+            c = mfma16(af[2][si], bf[0], c, FMT);
+            c = mfma16(af[0][si], bf[1], c, FMT);
+            c = mfma16(af[1][si], bf[0], c, FMT);
           }
+          acc[si][sj] = mfma16(af[0][si], bf[0], c, FMT);
         }
+      }
     }
   }
 }
@@ -324,15 +329,13 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
     soff[i] = r * ROW + cc;
   }
 
-  f32x4 acc[NP][TL::NACC][SUB][SUB];
+  f32x4 acc[NP][SUB][SUB];
 #pragma unroll
   for (int u = 0; u < NP; ++u)
 #pragma unroll
-    for (int q = 0; q < TL::NACC; ++q)
+    for (int a = 0; a < SUB; ++a)
 #pragma unroll
-      for (int a = 0; a < SUB; ++a)
-#pragma unroll
-        for (int b = 0; b < SUB; ++b) acc[u][q][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int b = 0; b < SUB; ++b) acc[u][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int arow = (wr * TL::WT + (lane & 15)) * ROW;
   const int brow = (wc * TL::WT + (lane & 15)) * ROW;
@@ -356,7 +359,8 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
   {                                                                                            \
     const E* base_ = smem + (buf) * TL::STAGE;                                                 \
     mma_slab<FMT, SUB>(base_, base_ + P * TILE, arow, brow, lane, acc[0]);                     \
-    if constexpr (AVG) mma_slab<FMT, SUB>(base_ + 2 * P * TILE, base_ + 3 * P * TILE, arow, brow, lane, acc[NP - 1]); \
+    if constexpr (AVG)                                                                         \
+      mma_slab<FMT, SUB>(base_ + 2 * P * TILE, base_ + 3 * P * TILE, arow, brow, lane, acc[NP - 1]); \
   }
   PSDP_GLOAD(0)
   PSDP_LSTORE(0)
@@ -375,28 +379,300 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
 
   // ---------------------------------------------------------------- epilogue
   // Accumulators go to LDS with compile-time indices only (they stay in
-  // registers), then the tile is written out by epilogue_from_lds.
+  // registers), HR rows at a time, then epilogue_rows writes them out.
+  constexpr float PSCALE = FMT == FMT_F16X3 ? F16X3_PROD_SCALE : 1.0f;
   float* Ct = (float*)smem;
   constexpr int CS = TL::BT + 1;
+  constexpr int PHASES = TL::BT / TL::HR;
 #pragma unroll
-  for (int si = 0; si < SUB; ++si)
+  for (int ph = 0; ph < PHASES; ++ph) {
+    // with two phases, wave row wr owns tile rows [wr*WT, wr*WT + WT) = phase wr
+    if (PHASES == 1 || wr == ph) {
 #pragma unroll
-    for (int sj = 0; sj < SUB; ++sj)
+      for (int si = 0; si < SUB; ++si)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float v = acc[0][0][si][sj][e];
-        if constexpr (TL::NACC == 2) v = fmaf(acc[0][1][si][sj][e], F16X3_LO_INV, v);
-        if constexpr (AVG) {
-          float w = acc[NP - 1][0][si][sj][e];
-          if constexpr (TL::NACC == 2) w = fmaf(acc[NP - 1][1][si][sj][e], F16X3_LO_INV, w);
-          v = 0.5f * (v + w);
+        for (int sj = 0; sj < SUB; ++sj)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float v = acc[0][si][sj][e];
+            if constexpr (AVG) v = 0.5f * (v + acc[NP - 1][si][sj][e]);
+            const int r = wr * TL::WT + si * 16 + (lane >> 4) * 4 + e - ph * TL::HR;
+            const int c = wc * TL::WT + sj * 16 + (lane & 15);
+            Ct[r * CS + c] = v * PSCALE;
+          }
+    }
+    __syncthreads();
+    epilogue_rows<TL::BT>(d, bi, bj, Ct, ph * TL::HR, TL::HR);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direct-to-LDS pipeline (16-bit formats).  Operand slabs go from global
+// memory straight into LDS (global_load_lds_dwordx4: lane L of a wave writes
+// bytes [16 L, 16 L + 16) of a 1 KB block whose base is wave-uniform), S
+// stages deep, so that S - 1 slabs are in flight while one is consumed; no
+// staging registers.  Rows are unpadded and their 16-byte chunks are XOR
+// swizzled per row (swz below) so that the 16-byte fragment reads of 16
+// consecutive rows are spread over all banks, for 32- and 64-bank LDS alike.
+// The loop waits for its own loads with an explicit vmcnt and crosses a bare
+// s_barrier (a __syncthreads would also wait for the slabs still in flight).
+template <int FMT, int SUB>
+struct DmaTiling {
+  static constexpr int P = FmtTraits<FMT>::P;
+  static constexpr int WT = 16 * SUB;
+  static constexpr int BT = 2 * WT;
+  // Small tiles keep LDS under 80 KB so that two workgroups share a CU.
+  static constexpr int STAGE_TARGET = SUB <= 2 ? 16384 : 32768;
+  static constexpr int LDS_BUDGET = SUB <= 2 ? 81920 : 163840;
+  static constexpr int BKR = STAGE_TARGET / (4 * P * BT);
+  static constexpr int BK = BKR >= 64 ? 64 : 32;          // slab width (elements)
+  static constexpr int RB = BK * 2;                        // row bytes
+  static constexpr int CPR = RB / 16;                      // 16-byte chunks per row
+  static constexpr int TILE_BYTES = BT * RB;               // one (panel, piece)
+  static constexpr int STAGE_BYTES = 2 * P * TILE_BYTES;   // X rows of bi, Y rows of bj
+  static constexpr int SFIT = LDS_BUDGET / STAGE_BYTES;
+  static constexpr int S = SFIT >= 4 ? 4 : 3;              // pipeline depth
+  static constexpr int L = STAGE_BYTES / 4096;             // loads per thread per stage
+  static constexpr int BLOCKS = TILE_BYTES / 1024;         // 1 KB blocks per tile
+  static constexpr int HR = BT < 128 ? BT : 128;
+  static constexpr int LDS_MAIN = S * STAGE_BYTES;
+  static constexpr int LDS_CTILE = HR * (BT + 1) * 4;
+  static constexpr int LDS_BYTES = LDS_MAIN > LDS_CTILE ? LDS_MAIN : LDS_CTILE;
+  static_assert(STAGE_BYTES % 4096 == 0, "stage must be a whole number of 4 KB loads");
+  static_assert(CPR == 8 || CPR == 4, "swizzle defined for 64- and 128-byte rows");
+  static_assert(S <= SFIT && LDS_BYTES <= LDS_BUDGET, "pipeline does not fit in LDS");
+  static_assert((S - 1) * L <= 63, "vmcnt range");
+};
+
+template <int CPR>
+__device__ __forceinline__ int dma_swz(int r) {
+  if constexpr (CPR == 8) return (r & 7) ^ ((r >> 3) & 1);
+  else return ((r >> 1) ^ ((r >> 3) & 1)) & 3;  // bits 0-3 of r only: rows 16 apart share it
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N <= 63, "vmcnt range");
+  // s_waitcnt: vmcnt[3:0] | expcnt[6:4] = 7 | lgkmcnt[11:8] = 15 | vmcnt[5:4] at [15:14]
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
+
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+template <int FMT, int SUB>
+__device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, char* smem) {
+  using DT = DmaTiling<FMT, SUB>;
+  constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, CPR = DT::CPR, S = DT::S, L = DT::L;
+  constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t ld = d.ld;
+  const int row0 = bi * DT::BT, col0 = bj * DT::BT;
+
+  // This thread's L loads per stage.  Load i of a wave covers one 1 KB block
+  // (RPB rows) of one (panel, piece): its base row pointer is wave-uniform
+  // (scalar registers); the lane part is a 32-bit offset that places logical
+  // chunk c of row r at physical chunk c ^ swz(r).  swz depends on r mod 16
+  // only, so the lane offset depends on the block parity when RPB = 8.
+  constexpr int RPB = 1024 / RB;  // rows per 1 KB block
+  const int rl = lane / CPR, pcl = lane % CPR;
+  int lane_off[2];
+#pragma unroll
+  for (int par = 0; par < 2; ++par) {
+    const int r16 = RPB == 16 ? rl : par * 8 + rl;
+    lane_off[par] = rl * (int)ld + (pcl ^ dma_swz<CPR>(r16)) * 8;
+  }
+  const uint16_t* gsrc[L];
+  int lofs[L], lpar[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int q = wave * L + i;
+    const int gp = q / DT::BLOCKS, blk = q % DT::BLOCKS;
+    const int g = gp / P, p = gp % P;
+    // select chain rather than a dynamic index: keeps a descriptor held in
+    // registers out of private memory
+    const void* bx = p == 0 ? d.x[0] : (p == 1 ? d.x[1] : d.x[2]);
+    const void* by = p == 0 ? d.y[0] : (p == 1 ? d.y[1] : d.y[2]);
+    const void* base = g == 0 ? bx : by;
+    gsrc[i] = (const uint16_t*)base + (int64_t)((g == 0 ? row0 : col0) + blk * RPB) * ld;
+    lofs[i] = gp * TB + blk * 1024;
+    lpar[i] = RPB == 16 ? 0 : (blk & 1);
+  }
+#define PSDP_DMA_ISSUE(K0, BUF)                                                                      \
+  _Pragma("unroll") for (int i = 0; i < L; ++i)                                                     \
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (K0) + (lpar[i] ? lane_off[1] : lane_off[0])), \
+                                       (lds_ptr_t)(smem + (BUF) * SB + lofs[i]), 16, 0, 0);
+
+  f32x4 acc[SUB][SUB];
+#pragma unroll
+  for (int a = 0; a < SUB; ++a)
+#pragma unroll
+    for (int b = 0; b < SUB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int swzl = dma_swz<CPR>(lane & 15);
+  const int arow = (wr * DT::WT + (lane & 15)) * RB;
+  const int brow = (wc * DT::WT + (lane & 15)) * RB;
+  const int KT = d.npad / BK;
+
+  // Fragments of one slab: KK k-steps of 32, P pieces, SUB row blocks.
+  // fa/fb hold the slab being multiplied; na receives the A fragments of the
+  // next slab, and the B fragments of the next slab overwrite fb column block
+  // by column block as soon as that block's MFMAs have been issued.
+  constexpr int KK = BK / 32;
+  u32x4 fa[KK][P][SUB], fb[KK][P][SUB], na[KK][P][SUB];
+  auto frag_a = [&](const char* stg, int kk, int p, int s2) -> u32x4 {
+    return *(const u32x4*)(stg + p * TB + arow + s2 * 16 * RB + 16 * ((kk * 4 + (lane >> 4)) ^ swzl));
+  };
+  auto frag_b = [&](const char* stg, int kk, int p, int s2) -> u32x4 {
+    return *(const u32x4*)(stg + (P + p) * TB + brow + s2 * 16 * RB + 16 * ((kk * 4 + (lane >> 4)) ^ swzl));
+  };
+
+  // Pipeline: S slab buffers.  Iteration kt computes slab kt from registers
+  // while slab kt + 1 is read from LDS and slabs kt + 2 .. kt + S are in
+  // flight from global memory; the DMA for slab kt + S reuses the buffer of
+  // slab kt, whose fragments every wave holds in registers by then.
+  // Every iteration issues one stage (past the end: a copy of the last slab
+  // into a buffer that is not read again), so that the wait counts are
+  // uniform and the loop body is one block that the scheduler can interleave.
+#pragma unroll
+  for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((st < KT ? st : KT - 1) * BK, st) }
+  wait_vmcnt<(S - 1) * L>();
+  lds_barrier();
+  {
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int s2 = 0; s2 < SUB; ++s2) {
+          fa[kk][p][s2] = frag_a(smem, kk, p, s2);
+          fb[kk][p][s2] = frag_b(smem, kk, p, s2);
         }
-        const int r = wr * TL::WT + si * 16 + (lane >> 4) * 4 + e;
-        const int c = wc * TL::WT + sj * 16 + (lane & 15);
-        Ct[r * CS + c] = v;
+  }
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+
+  // MFMAs of column block sj of k-step kk
+#define PSDP_COLUMN_MMA(kk, sj)                                                                   \
+  _Pragma("unroll") for (int si = 0; si < SUB; ++si) {                                            \
+    f32x4 c = acc[si][sj];                                                                        \
+    if constexpr (P == 2) { /* x3: lo*hi, hi*lo, then hi*hi */                                    \
+      c = mfma16(fa[kk][1][si], fb[kk][0][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][0][si], fb[kk][1][sj], c, FMT);                                           \
+    } else if constexpr (P == 3) { /* x9: smallest terms first */                                 \
+      c = mfma16(fa[kk][2][si], fb[kk][2][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][1][si], fb[kk][2][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][2][si], fb[kk][1][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][1][si], fb[kk][1][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][0][si], fb[kk][2][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][2][si], fb[kk][0][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][0][si], fb[kk][1][sj], c, FMT);                                           \
+      c = mfma16(fa[kk][1][si], fb[kk][0][sj], c, FMT);                                           \
+    }                                                                                             \
+    acc[si][sj] = mfma16(fa[kk][0][si], fb[kk][0][sj], c, FMT);                                   \
+  }
+
+  constexpr int NREAD = 2 * KK * P * SUB;  // LDS reads per slab
+  constexpr int NMFMA = KK * SUB * SUB * (P == 1 ? 1 : (P == 2 ? 3 : 9));
+  constexpr int MR = NMFMA / NREAD > 0 ? NMFMA / NREAD : 1;
+  static_assert(L <= NREAD, "one global->LDS load per LDS-read group");
+  for (int kt = 0; kt + 1 < KT; ++kt) {
+    // own reads of buffer kt % S complete, own DMA of slab kt + 1 landed
+    wait_vmcnt<(S - 2) * L>();
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    lds_barrier();
+    PSDP_DMA_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S)
+    const char* nxt = smem + ((kt + 1) % S) * SB;
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int s2 = 0; s2 < SUB; ++s2) na[kk][p][s2] = frag_a(nxt, kk, p, s2);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+      for (int sj = 0; sj < SUB; ++sj) {
+        PSDP_COLUMN_MMA(kk, sj)
+#pragma unroll
+        for (int p = 0; p < P; ++p) fb[kk][p][sj] = frag_b(nxt, kk, p, sj);
       }
-  __syncthreads();
-  epilogue_from_lds<TL::BT>(d, bi, bj, Ct);
+    // Interleave: (one global->LDS load,) one LDS read, then MR MFMAs, so
+    // that the memory pipes work while the matrix cores do (issuing all
+    // reads first stalls the wave on the LDS command queue with the matrix
+    // cores idle).
+#pragma unroll
+    for (int g = 0; g < NREAD; ++g) {
+      if (g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      __builtin_amdgcn_sched_group_barrier(0x008, MR, 0);  // MFMA
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int s2 = 0; s2 < SUB; ++s2) fa[kk][p][s2] = na[kk][p][s2];
+  }
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+    for (int sj = 0; sj < SUB; ++sj) { PSDP_COLUMN_MMA(kk, sj) }
+#undef PSDP_COLUMN_MMA
+  __builtin_amdgcn_sched_barrier(0);  // keep the epilogue's accumulator reads after the MFMAs
+  wait_vmcnt<0>();  // trailing stage copies land before the epilogue reuses LDS
+#undef PSDP_DMA_ISSUE
+  lds_barrier();  // every wave is done reading the stages before they hold C
+
+  // ---------------------------------------------------------------- epilogue
+  constexpr float PSCALE = FMT == FMT_F16X3 ? F16X3_PROD_SCALE : 1.0f;
+  float* Ct = (float*)smem;
+  constexpr int CS = DT::BT + 1;
+  constexpr int PHASES = DT::BT / DT::HR;
+#pragma unroll
+  for (int ph = 0; ph < PHASES; ++ph) {
+    if (PHASES == 1 || wr == ph) {
+#pragma unroll
+      for (int si = 0; si < SUB; ++si)
+#pragma unroll
+        for (int sj = 0; sj < SUB; ++sj)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = wr * DT::WT + si * 16 + (lane >> 4) * 4 + e - ph * DT::HR;
+            const int c = wc * DT::WT + sj * 16 + (lane & 15);
+            Ct[r * CS + c] = acc[si][sj][e] * PSCALE;
+          }
+    }
+    __syncthreads();
+    epilogue_rows<DT::BT>(d, bi, bj, Ct, ph * DT::HR, DT::HR);
+    __syncthreads();
+  }
+}
+
+template <int FMT, int SUB>
+__global__ __launch_bounds__(256) void sym_prod_dma_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB>::LDS_BYTES];
+  const TileRef t = tiles[blockIdx.x];
+  if (t.bi < 0) return;
+  sym_tile_dma<FMT, SUB>(d, t.bi, t.bj, smem);
+}
+
+template <int FMT, int SUB>
+__global__ __launch_bounds__(256) void sym_prod_dma_grouped_kernel(const ProdDesc* __restrict__ descs,
+                                                                   const TileRef* __restrict__ tiles, int ntiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB>::LDS_BYTES];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    sym_tile_dma<FMT, SUB>(descs[tr.mat], tr.bi, tr.bj, smem);  // descriptor read in place
+  }
 }
 
 // One workgroup per upper-triangle tile of a single matrix.
@@ -421,7 +697,6 @@ __global__ __launch_bounds__(256) void sym_prod_grouped_kernel(const ProdDesc* _
     const TileRef tr = tiles[t];
     const ProdDesc d = descs[tr.mat];
     sym_tile<FMT, SUB, AVG>(d, tr.bi, tr.bj, smem);
-    __syncthreads();
   }
 }
 
@@ -439,62 +714,80 @@ static hipError_t launch_grouped(const ProdDesc* descs, const TileRef* tiles, in
   return hipGetLastError();
 }
 
-int sym_tile_size(int sub) { return 32 * sub; }
+// 256 x 256 tiles (SUB = 8) exist for the 1-piece 16-bit formats with plain
+// (non-averaged) products.
+bool sym_big_tile_capable(int fmt, int avg) { return !avg && (fmt == FMT_F16 || fmt == FMT_BF16); }
+
+template <int FMT, int SUB>
+static hipError_t launch_dma(const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
+  hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, SUB>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+  return hipGetLastError();
+}
+
+template <int FMT, int SUB>
+static hipError_t launch_dma_grouped(const ProdDesc* descs, const TileRef* tiles, int ntiles, int grid,
+                                     hipStream_t s) {
+  hipLaunchKernelGGL((sym_prod_dma_grouped_kernel<FMT, SUB>), dim3(grid), dim3(256), 0, s, descs, tiles, ntiles);
+  return hipGetLastError();
+}
 
 // avg selects the averaged (Jordan) product; it exists for the 1-piece
 // 16-bit formats, whose operand rounding makes C visibly non-symmetric.
+// Plain 16-bit products use the direct-to-LDS kernels; FP32 and averaged
+// products use the register-staged ones.
 hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, const TileRef* tiles, int ntiles,
                               hipStream_t s) {
-#define PSDP_CASE(F)                                                    \
-  case F:                                                               \
-    return sub == 4 ? launch_one<F, 4, false>(d, tiles, ntiles, s)      \
-                    : launch_one<F, 2, false>(d, tiles, ntiles, s);
   if (avg) {
     if (fmt == FMT_F16)
       return sub == 4 ? launch_one<FMT_F16, 4, true>(d, tiles, ntiles, s)
-                      : launch_one<FMT_F16, 2, true>(d, tiles, ntiles, s);
+                      : (sub == 2 ? launch_one<FMT_F16, 2, true>(d, tiles, ntiles, s) : hipErrorInvalidValue);
     if (fmt == FMT_BF16)
       return sub == 4 ? launch_one<FMT_BF16, 4, true>(d, tiles, ntiles, s)
-                      : launch_one<FMT_BF16, 2, true>(d, tiles, ntiles, s);
+                      : (sub == 2 ? launch_one<FMT_BF16, 2, true>(d, tiles, ntiles, s) : hipErrorInvalidValue);
+    return hipErrorInvalidValue;
   }
+#define PSDP_CASE1(F)                                                           \
+  case F:                                                                       \
+    return sub == 8 ? launch_dma<F, 8>(d, tiles, ntiles, s)                     \
+                    : (sub == 4 ? launch_dma<F, 4>(d, tiles, ntiles, s)         \
+                                : launch_dma<F, 2>(d, tiles, ntiles, s));
+#define PSDP_CASEN(F)                                                           \
+  case F:                                                                       \
+    return sub == 4 ? launch_dma<F, 4>(d, tiles, ntiles, s)                     \
+                    : (sub == 2 ? launch_dma<F, 2>(d, tiles, ntiles, s) : hipErrorInvalidValue);
   switch (fmt) {
-    PSDP_CASE(FMT_F32)
-    PSDP_CASE(FMT_F16)
-    PSDP_CASE(FMT_F16X3)
-    PSDP_CASE(FMT_BF16)
-    PSDP_CASE(FMT_BF16X3)
-    PSDP_CASE(FMT_BF16X9)
+    case FMT_F32:
+      return sub == 4 ? launch_one<FMT_F32, 4, false>(d, tiles, ntiles, s)
+                      : (sub == 2 ? launch_one<FMT_F32, 2, false>(d, tiles, ntiles, s) : hipErrorInvalidValue);
+    PSDP_CASE1(FMT_F16)
+    PSDP_CASE1(FMT_BF16)
+    PSDP_CASEN(FMT_F16X3)
+    PSDP_CASEN(FMT_BF16X3)
+    PSDP_CASEN(FMT_BF16X9)
     default:
       return hipErrorInvalidValue;
   }
-#undef PSDP_CASE
+#undef PSDP_CASE1
+#undef PSDP_CASEN
 }
 
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,
                                       int ntiles, int grid, hipStream_t s) {
-#define PSDP_CASE(F)                                                             \
-  case F:                                                                        \
-    return sub == 4 ? launch_grouped<F, 4, false>(descs, tiles, ntiles, grid, s) \
-                    : launch_grouped<F, 2, false>(descs, tiles, ntiles, grid, s);
+  if (sub != 2) return hipErrorInvalidValue;
   if (avg) {
-    if (fmt == FMT_F16)
-      return sub == 4 ? launch_grouped<FMT_F16, 4, true>(descs, tiles, ntiles, grid, s)
-                      : launch_grouped<FMT_F16, 2, true>(descs, tiles, ntiles, grid, s);
-    if (fmt == FMT_BF16)
-      return sub == 4 ? launch_grouped<FMT_BF16, 4, true>(descs, tiles, ntiles, grid, s)
-                      : launch_grouped<FMT_BF16, 2, true>(descs, tiles, ntiles, grid, s);
+    if (fmt == FMT_F16) return launch_grouped<FMT_F16, 2, true>(descs, tiles, ntiles, grid, s);
+    if (fmt == FMT_BF16) return launch_grouped<FMT_BF16, 2, true>(descs, tiles, ntiles, grid, s);
+    return hipErrorInvalidValue;
   }
   switch (fmt) {
-    PSDP_CASE(FMT_F32)
-    PSDP_CASE(FMT_F16)
-    PSDP_CASE(FMT_F16X3)
-    PSDP_CASE(FMT_BF16)
-    PSDP_CASE(FMT_BF16X3)
-    PSDP_CASE(FMT_BF16X9)
-    default:
-      return hipErrorInvalidValue;
+    case FMT_F32: return launch_grouped<FMT_F32, 2, false>(descs, tiles, ntiles, grid, s);
+    case FMT_F16: return launch_dma_grouped<FMT_F16, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_F16X3: return launch_dma_grouped<FMT_F16X3, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_BF16: return launch_dma_grouped<FMT_BF16, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 2>(descs, tiles, ntiles, grid, s);
+    default: return hipErrorInvalidValue;
   }
-#undef PSDP_CASE
 }
 
 }  // namespace psdp

@@ -47,7 +47,7 @@ def test_default_config_and_workspace():
     assert cfg.auto_scale == 1 and cfg.lanczos_iters == 20 and cfg.lanczos_tol == 1e-10
     ws_mixed = P.workspace_size(16384, "mixed")
     ws_fp32 = P.workspace_size(16384, "composite_FP32")
-    N = 16384 * 16384
+    N = 16384 * (16384 + 64)  # rows padded by 64 elements
     # FP32 chain: A, A^3, A^2, X_s in float32 (+ Lanczos vectors)
     assert 16 * N <= ws_fp32 < 16 * N + (8 << 20)
     # mixed: A, A^3 float32 + 2 binary16 pieces for each of A, A^2, A^3, X_s

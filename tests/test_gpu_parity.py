@@ -55,7 +55,7 @@ def start_vector(seed, n):
 # ------------------------------------------------------------------ products
 @pytest.mark.parametrize("fmt,avg", [("f32", 0), ("f16", 0), ("f16x3", 0), ("bf16", 0), ("bf16x3", 0),
                                      ("bf16x9", 0), ("f16", 1), ("bf16", 1)])
-@pytest.mark.parametrize("n", [64, 200, 1100])
+@pytest.mark.parametrize("n", [64, 200, 1100, 2100])
 def test_sym_product(handle, fmt, avg, n):
     # The kernel computes tiles with bi <= bj only; its result is the upper
     # triangle of X Y mirrored (or, averaged, (X Y + Y X)/2).  Compared with
@@ -195,6 +195,7 @@ def test_reference_uniform_autoscaled_1024(handle, fn, method):
     ("mixed", "cfg2", 1024, 1e-3),
     ("mixed", "cfg3", 1024, 1e-3),
     ("mixed", "uniform", 640, 1e-3),
+    ("mixed", "cfg3", 2100, 1e-3),
 ])
 def test_projection_accuracy(handle, method, kind, n, bar):
     print(f"projection {method} {kind} n={n}", flush=True)

@@ -61,7 +61,7 @@ def test_split_reconstructs(fmt, bits):
     x = (rng.standard_normal(10000) * 0.01).astype(np.float32)
     p = O.split_operand(x, fmt)
     if fmt == "f16x3":
-        r = p[0].astype(np.float64) + p[1].astype(np.float64) / O.F16X3_LO_SCALE
+        r = (p[0].astype(np.float64) + p[1].astype(np.float64)) / O.F16X3_SCALE
     else:
         r = sum(q.astype(np.float64) for q in p)
     # absolute error relative to the magnitude of the operand (tiny entries
