@@ -52,6 +52,20 @@ def cpu_baseline(X64_host_block, full_n):
             "measured_s": dt, "sample_n": m}
 
 
+def measured_traffic(n, fmt):
+    """HBM bytes per launch of the dominant kernel, from the PMC summary committed
+    next to this file (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain copy);
+    None when no summary for this size and format is present."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        rec = t.get(f"n{n}_{fmt}")
+        return None if rec is None else rec["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +75,10 @@ def main():
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=6144, help="CPU baseline block size (0: skip)")
+    ap.add_argument("--basis", default="haar", choices=("haar", "householder"),
+                    help="eigenvector basis of the synthetic input (householder: no QR, for counter runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one plain 1 GiB device copy first (scale reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the projection of one matrix does not shard "
@@ -76,7 +94,7 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    X64, Pm, lam = W.spectral_matrix(n, 3, "cfg3")
+    X64, Pm, lam = W.spectral_matrix(n, 3, "cfg3", basis=args.basis)
     X = X64.float().contiguous()
     torch.cuda.synchronize()
     log(f"workload n={n} generated in {time.perf_counter() - t0:.1f} s")
@@ -85,6 +103,15 @@ def main():
         m = min(args.cpu_sample, n)
         cpu_block = X64[:m, :m].cpu().numpy().astype(np.float64)
     del X64
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, device=dev)
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 1 GiB read + 1 GiB written")
 
     h = P.Handle(0, seed=2001)
     Y = torch.empty_like(X)
@@ -120,19 +147,25 @@ def main():
     flops = PRODUCTS[fmt] * wsym
     achieved = flops / (avg_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[fmt],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4), "traffic": None,
-                "kernel": f"sym_prod_kernel<{fmt}>", "launches": cnt, "avg_ms": round(avg_ms, 4),
-                "flops_per_launch": flops}
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
+                "traffic": measured_traffic(n, fmt),
+                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel)", "launches": cnt,
+                "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
+                "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
     total_chain_flops = sum(PRODUCTS[f] * wsym * c for f, (c, _) in prods.items()) / args.steps
     chain_ms = sum(t for _, t in prods.values()) / args.steps
 
     out = {"metric": METRIC, "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None,
-           "dtype": "f16 MFMA operands (hi/lo f16 split on steps 0-2), f32 accumulate; f64 Lanczos"
-           if args.method == "mixed" else args.method,
-           "data": "synthetic: Haar Q (QR of a seeded Gaussian, torch FP64 on the GPU), lambda ~ U[-5,5] "
-                   "with 10% in [-1e-3,1e-3]; float32 storage",
+           "dtype": "f16" if args.method in ("mixed", "composite_FP16") else
+           ("f32" if args.method == "composite_FP32" else args.method),
+           "dtype_note": "binary16 MFMA operands with f32 accumulation; steps 0-3 and the final product "
+                         "use a 3-term hi/lo binary16 split (FP32-level products); Lanczos in f64"
+           if args.method == "mixed" else "",
+           "data": ("synthetic: Haar Q (QR of a seeded Gaussian, torch FP64 on the GPU)" if args.basis == "haar"
+                    else "synthetic: Householder Q (one seeded reflector)") +
+                   ", lambda ~ U[-5,5] with 10% in [-1e-3,1e-3]; float32 storage",
            "config": {"workload": f"single n={n} projection (config 3 spectrum), {args.method} schedule, "
                                   "Lanczos 20 it + 7-step FP16-table chain + final product",
                       "n": n, "method": args.method},

@@ -85,6 +85,20 @@ struct Tiling {
   static_assert(LDS_BYTES <= 163840, "LDS budget of one workgroup exceeded");
 };
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// 32x32x16 MFMA: lane l holds row (column) l % 32 and k = 8 (l / 32) .. +7 of
+// A (B); result element e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4,
+// column l % 32.  One wave reaches the matrix-core peak with this shape,
+// which it does not with 16x16x32.
+__device__ __forceinline__ f32x16 mfma32(u32x4 a, u32x4 b, f32x16 c, int fmt) {
+  if (fmt == FMT_BF16 || fmt == FMT_BF16X3 || fmt == FMT_BF16X9)
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
 __device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c, int fmt) {
   if (fmt == FMT_BF16 || fmt == FMT_BF16X3 || fmt == FMT_BF16X9)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
@@ -412,9 +426,15 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
 // memory straight into LDS (global_load_lds_dwordx4: lane L of a wave writes
 // bytes [16 L, 16 L + 16) of a 1 KB block whose base is wave-uniform), S
 // stages deep, so that S - 1 slabs are in flight while one is consumed; no
-// staging registers.  Rows are unpadded and their 16-byte chunks are XOR
-// swizzled per row (swz below) so that the 16-byte fragment reads of 16
-// consecutive rows are spread over all banks, for 32- and 64-bank LDS alike.
+// staging registers.  Slab rows are unpadded (RB = 64 or 128 bytes) and
+// their 16-byte chunks are XOR swizzled: logical chunk c of row r sits at
+// physical chunk c ^ swz(r), swz(r) = (r >> 2) & 3 for 64-byte rows and
+// (r >> 1) & 7 for 128-byte rows.  A 32x32x16 fragment read (lane l: row
+// l % 32, chunk 2 ks + l / 32) is then free of LDS bank conflicts; both swizzles were
+// chosen by measurement (SQ_LDS_BANK_CONFLICT over candidate patterns),
+// several XOR patterns that look conflict-free under a plain 64-bank model
+// cost two cycles per read here.  Each global->LDS load reads whole 64- or
+// 128-byte row segments with consecutive lanes, as coalescing wants.
 // The loop waits for its own loads with an explicit vmcnt and crosses a bare
 // s_barrier (a __syncthreads would also wait for the slabs still in flight).
 template <int FMT, int SUB>
@@ -423,16 +443,16 @@ struct DmaTiling {
   static constexpr int WT = 16 * SUB;
   static constexpr int BT = 2 * WT;
   // Small tiles keep LDS under 80 KB so that two workgroups share a CU.
-  static constexpr int STAGE_TARGET = SUB <= 2 ? 16384 : 32768;
   static constexpr int LDS_BUDGET = SUB <= 2 ? 81920 : 163840;
-  static constexpr int BKR = STAGE_TARGET / (4 * P * BT);
-  static constexpr int BK = BKR >= 64 ? 64 : 32;          // slab width (elements)
+  // 64-element slabs make every row segment a whole 128-byte line (half as
+  // many L2 requests as 32-element slabs); three-piece operands would not
+  // fit two such stages.
+  static constexpr int BK = P == 3 ? 32 : 64;              // slab width (elements)
   static constexpr int RB = BK * 2;                        // row bytes
-  static constexpr int CPR = RB / 16;                      // 16-byte chunks per row
   static constexpr int TILE_BYTES = BT * RB;               // one (panel, piece)
   static constexpr int STAGE_BYTES = 2 * P * TILE_BYTES;   // X rows of bi, Y rows of bj
   static constexpr int SFIT = LDS_BUDGET / STAGE_BYTES;
-  static constexpr int S = SFIT >= 4 ? 4 : 3;              // pipeline depth
+  static constexpr int S = SFIT >= 4 ? 4 : SFIT;           // pipeline depth (>= 2)
   static constexpr int L = STAGE_BYTES / 4096;             // loads per thread per stage
   static constexpr int BLOCKS = TILE_BYTES / 1024;         // 1 KB blocks per tile
   static constexpr int HR = BT < 128 ? BT : 128;
@@ -440,15 +460,17 @@ struct DmaTiling {
   static constexpr int LDS_CTILE = HR * (BT + 1) * 4;
   static constexpr int LDS_BYTES = LDS_MAIN > LDS_CTILE ? LDS_MAIN : LDS_CTILE;
   static_assert(STAGE_BYTES % 4096 == 0, "stage must be a whole number of 4 KB loads");
-  static_assert(CPR == 8 || CPR == 4, "swizzle defined for 64- and 128-byte rows");
-  static_assert(S <= SFIT && LDS_BYTES <= LDS_BUDGET, "pipeline does not fit in LDS");
+  static_assert(RB == 64 || RB == 128, "swizzle defined for 64- and 128-byte rows");
+  static_assert(S >= 2 && S <= SFIT && LDS_BYTES <= LDS_BUDGET, "pipeline does not fit in LDS");
   static_assert((S - 1) * L <= 63, "vmcnt range");
 };
 
-template <int CPR>
+// Chunk swizzle of slab row r (see the layout note above); depends on r mod 32
+// only, so the 32-row blocks of one wave share it.
+template <int RB>
 __device__ __forceinline__ int dma_swz(int r) {
-  if constexpr (CPR == 8) return (r & 7) ^ ((r >> 3) & 1);
-  else return ((r >> 1) ^ ((r >> 3) & 1)) & 3;  // bits 0-3 of r only: rows 16 apart share it
+  if constexpr (RB == 64) return (r >> 2) & 3;
+  else return (r >> 1) & 7;
 }
 
 template <int N>
@@ -469,7 +491,7 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 template <int FMT, int SUB>
 __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, char* smem) {
   using DT = DmaTiling<FMT, SUB>;
-  constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, CPR = DT::CPR, S = DT::S, L = DT::L;
+  constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, S = DT::S, L = DT::L;
   constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
@@ -477,21 +499,13 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   const int64_t ld = d.ld;
   const int row0 = bi * DT::BT, col0 = bj * DT::BT;
 
-  // This thread's L loads per stage.  Load i of a wave covers one 1 KB block
-  // (RPB rows) of one (panel, piece): its base row pointer is wave-uniform
-  // (scalar registers); the lane part is a 32-bit offset that places logical
-  // chunk c of row r at physical chunk c ^ swz(r).  swz depends on r mod 16
-  // only, so the lane offset depends on the block parity when RPB = 8.
-  constexpr int RPB = 1024 / RB;  // rows per 1 KB block
-  const int rl = lane / CPR, pcl = lane % CPR;
-  int lane_off[2];
-#pragma unroll
-  for (int par = 0; par < 2; ++par) {
-    const int r16 = RPB == 16 ? rl : par * 8 + rl;
-    lane_off[par] = rl * (int)ld + (pcl ^ dma_swz<CPR>(r16)) * 8;
-  }
+  // This thread's L loads per stage.  Load i of a wave fills one 1 KB block
+  // (RPB whole rows) of one (panel, piece); its base pointer is wave-uniform
+  // (scalar registers).  Lane l takes row l / CPR of the block and physical
+  // chunk l % CPR, i.e. logical chunk (l % CPR) ^ swz(row).
+  constexpr int CPR = RB / 16, RPB = 1024 / RB;
   const uint16_t* gsrc[L];
-  int lofs[L], lpar[L];
+  int lofs[L], loff[L];
 #pragma unroll
   for (int i = 0; i < L; ++i) {
     const int q = wave * L + i;
@@ -504,35 +518,40 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     const void* base = g == 0 ? bx : by;
     gsrc[i] = (const uint16_t*)base + (int64_t)((g == 0 ? row0 : col0) + blk * RPB) * ld;
     lofs[i] = gp * TB + blk * 1024;
-    lpar[i] = RPB == 16 ? 0 : (blk & 1);
+    const int r = blk * RPB + lane / CPR;
+    loff[i] = (lane / CPR) * (int)ld + ((lane % CPR) ^ dma_swz<RB>(r)) * 8;
   }
 #define PSDP_DMA_ISSUE(K0, BUF)                                                                      \
   _Pragma("unroll") for (int i = 0; i < L; ++i)                                                     \
-      __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (K0) + (lpar[i] ? lane_off[1] : lane_off[0])), \
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (K0) + loff[i]),                     \
                                        (lds_ptr_t)(smem + (BUF) * SB + lofs[i]), 16, 0, 0);
 
-  f32x4 acc[SUB][SUB];
+  // Each wave: M x M blocks of 32 x 32 (M = SUB / 2), KS k-steps of 16 per slab.
+  constexpr int M = SUB / 2, KS = BK / 16;
+  static_assert(SUB % 2 == 0, "32-row blocks");
+  f32x16 acc[M][M];
 #pragma unroll
-  for (int a = 0; a < SUB; ++a)
+  for (int a = 0; a < M; ++a)
 #pragma unroll
-    for (int b = 0; b < SUB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
 
-  const int swzl = dma_swz<CPR>(lane & 15);
-  const int arow = (wr * DT::WT + (lane & 15)) * RB;
-  const int brow = (wc * DT::WT + (lane & 15)) * RB;
+  // the lane's row within this wave's A and B row blocks
+  const int arow = (wr * DT::WT + (lane & 31)) * RB;
+  const int brow = (wc * DT::WT + (lane & 31)) * RB;
+  const int swzl = dma_swz<RB>(lane & 31);
   const int KT = d.npad / BK;
 
-  // Fragments of one slab: KK k-steps of 32, P pieces, SUB row blocks.
   // fa/fb hold the slab being multiplied; na receives the A fragments of the
   // next slab, and the B fragments of the next slab overwrite fb column block
   // by column block as soon as that block's MFMAs have been issued.
-  constexpr int KK = BK / 32;
-  u32x4 fa[KK][P][SUB], fb[KK][P][SUB], na[KK][P][SUB];
-  auto frag_a = [&](const char* stg, int kk, int p, int s2) -> u32x4 {
-    return *(const u32x4*)(stg + p * TB + arow + s2 * 16 * RB + 16 * ((kk * 4 + (lane >> 4)) ^ swzl));
+  u32x4 fa[KS][P][M], fb[KS][P][M], na[KS][P][M];
+  auto frag_a = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + p * TB + arow + m * 32 * RB + 16 * ((ks * 2 + (lane >> 5)) ^ swzl));
   };
-  auto frag_b = [&](const char* stg, int kk, int p, int s2) -> u32x4 {
-    return *(const u32x4*)(stg + (P + p) * TB + brow + s2 * 16 * RB + 16 * ((kk * 4 + (lane >> 4)) ^ swzl));
+  auto frag_b = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + (P + p) * TB + brow + m * 32 * RB + 16 * ((ks * 2 + (lane >> 5)) ^ swzl));
   };
 
   // Pipeline: S slab buffers.  Iteration kt computes slab kt from registers
@@ -546,43 +565,40 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((st < KT ? st : KT - 1) * BK, st) }
   wait_vmcnt<(S - 1) * L>();
   lds_barrier();
-  {
 #pragma unroll
-    for (int kk = 0; kk < KK; ++kk)
+  for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-      for (int p = 0; p < P; ++p)
+    for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int s2 = 0; s2 < SUB; ++s2) {
-          fa[kk][p][s2] = frag_a(smem, kk, p, s2);
-          fb[kk][p][s2] = frag_b(smem, kk, p, s2);
-        }
-  }
+      for (int m = 0; m < M; ++m) {
+        fa[ks][p][m] = frag_a(smem, ks, p, m);
+        fb[ks][p][m] = frag_b(smem, ks, p, m);
+      }
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
 
-  // MFMAs of column block sj of k-step kk
-#define PSDP_COLUMN_MMA(kk, sj)                                                                   \
-  _Pragma("unroll") for (int si = 0; si < SUB; ++si) {                                            \
-    f32x4 c = acc[si][sj];                                                                        \
+  // MFMAs of column block mj of k-step ks
+#define PSDP_COLUMN_MMA(ks, mj)                                                                   \
+  _Pragma("unroll") for (int mi = 0; mi < M; ++mi) {                                              \
+    f32x16 c = acc[mi][mj];                                                                       \
     if constexpr (P == 2) { /* x3: lo*hi, hi*lo, then hi*hi */                                    \
-      c = mfma16(fa[kk][1][si], fb[kk][0][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][0][si], fb[kk][1][sj], c, FMT);                                           \
+      c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                           \
     } else if constexpr (P == 3) { /* x9: smallest terms first */                                 \
-      c = mfma16(fa[kk][2][si], fb[kk][2][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][1][si], fb[kk][2][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][2][si], fb[kk][1][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][1][si], fb[kk][1][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][0][si], fb[kk][2][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][2][si], fb[kk][0][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][0][si], fb[kk][1][sj], c, FMT);                                           \
-      c = mfma16(fa[kk][1][si], fb[kk][0][sj], c, FMT);                                           \
+      c = mfma32(fa[ks][2][mi], fb[ks][2][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][1][mi], fb[ks][2][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][2][mi], fb[ks][1][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][1][mi], fb[ks][1][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][0][mi], fb[ks][2][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][2][mi], fb[ks][0][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                           \
     }                                                                                             \
-    acc[si][sj] = mfma16(fa[kk][0][si], fb[kk][0][sj], c, FMT);                                   \
+    acc[mi][mj] = mfma32(fa[ks][0][mi], fb[ks][0][mj], c, FMT);                                   \
   }
 
-  constexpr int NREAD = 2 * KK * P * SUB;  // LDS reads per slab
-  constexpr int NMFMA = KK * SUB * SUB * (P == 1 ? 1 : (P == 2 ? 3 : 9));
+  constexpr int NREAD = 2 * KS * P * M;  // LDS reads per slab
+  constexpr int NMFMA = KS * M * M * (P == 1 ? 1 : (P == 2 ? 3 : 9));
   constexpr int MR = NMFMA / NREAD > 0 ? NMFMA / NREAD : 1;
-  static_assert(L <= NREAD, "one global->LDS load per LDS-read group");
   for (int kt = 0; kt + 1 < KT; ++kt) {
     // own reads of buffer kt % S complete, own DMA of slab kt + 1 landed
     wait_vmcnt<(S - 2) * L>();
@@ -591,23 +607,22 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     PSDP_DMA_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S)
     const char* nxt = smem + ((kt + 1) % S) * SB;
 #pragma unroll
-    for (int kk = 0; kk < KK; ++kk)
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int s2 = 0; s2 < SUB; ++s2) na[kk][p][s2] = frag_a(nxt, kk, p, s2);
+        for (int m = 0; m < M; ++m) na[ks][p][m] = frag_a(nxt, ks, p, m);
 #pragma unroll
-    for (int kk = 0; kk < KK; ++kk)
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-      for (int sj = 0; sj < SUB; ++sj) {
-        PSDP_COLUMN_MMA(kk, sj)
+      for (int mj = 0; mj < M; ++mj) {
+        PSDP_COLUMN_MMA(ks, mj)
 #pragma unroll
-        for (int p = 0; p < P; ++p) fb[kk][p][sj] = frag_b(nxt, kk, p, sj);
+        for (int p = 0; p < P; ++p) fb[ks][p][mj] = frag_b(nxt, ks, p, mj);
       }
-    // Interleave: (one global->LDS load,) one LDS read, then MR MFMAs, so
-    // that the memory pipes work while the matrix cores do (issuing all
-    // reads first stalls the wave on the LDS command queue with the matrix
-    // cores idle).
+    // Interleave global->LDS loads and LDS reads with the MFMAs so that the
+    // memory pipes work while the matrix cores do (issuing all reads first
+    // stalls the wave on the LDS command queue with the matrix cores idle).
 #pragma unroll
     for (int g = 0; g < NREAD; ++g) {
       if (g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
@@ -616,16 +631,16 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int kk = 0; kk < KK; ++kk)
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int s2 = 0; s2 < SUB; ++s2) fa[kk][p][s2] = na[kk][p][s2];
+        for (int m = 0; m < M; ++m) fa[ks][p][m] = na[ks][p][m];
   }
 #pragma unroll
-  for (int kk = 0; kk < KK; ++kk)
+  for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-    for (int sj = 0; sj < SUB; ++sj) { PSDP_COLUMN_MMA(kk, sj) }
+    for (int mj = 0; mj < M; ++mj) { PSDP_COLUMN_MMA(ks, mj) }
 #undef PSDP_COLUMN_MMA
   __builtin_amdgcn_sched_barrier(0);  // keep the epilogue's accumulator reads after the MFMAs
   wait_vmcnt<0>();  // trailing stage copies land before the epilogue reuses LDS
@@ -641,14 +656,14 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   for (int ph = 0; ph < PHASES; ++ph) {
     if (PHASES == 1 || wr == ph) {
 #pragma unroll
-      for (int si = 0; si < SUB; ++si)
+      for (int mi = 0; mi < M; ++mi)
 #pragma unroll
-        for (int sj = 0; sj < SUB; ++sj)
+        for (int mj = 0; mj < M; ++mj)
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int r = wr * DT::WT + si * 16 + (lane >> 4) * 4 + e - ph * DT::HR;
-            const int c = wc * DT::WT + sj * 16 + (lane & 15);
-            Ct[r * CS + c] = acc[si][sj][e] * PSCALE;
+          for (int e = 0; e < 16; ++e) {
+            const int r = wr * DT::WT + mi * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3) - ph * DT::HR;
+            const int c = wc * DT::WT + mj * 32 + (lane & 31);
+            Ct[r * CS + c] = acc[mi][mj][e] * PSCALE;
           }
     }
     __syncthreads();

@@ -45,11 +45,24 @@ def haar(n: int, seed: int, device="cuda"):
     return Q
 
 
-def spectral_matrix(n: int, seed: int, kind: str, device="cuda", want_projection=True):
-    """Returns (X float64, Pi(X) float64 or None, lam numpy)."""
+def householder(n: int, seed: int, device="cuda"):
+    """Q = I - 2 v v^T / (v^T v) for a seeded Gaussian v: exactly orthogonal and
+    built without a QR factorisation (a cheap stand-in for Haar Q where only
+    the shape and spectrum of the workload matter, e.g. for counter runs)."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    v = torch.randn(n, 1, dtype=torch.float64, device=device, generator=g)
+    Q = torch.eye(n, dtype=torch.float64, device=device)
+    Q.sub_((2.0 / float((v * v).sum())) * (v @ v.T))
+    return Q
+
+
+def spectral_matrix(n: int, seed: int, kind: str, device="cuda", want_projection=True, basis="haar"):
+    """Returns (X float64, Pi(X) float64 or None, lam numpy).  basis: "haar"
+    (QR of a seeded Gaussian with the sign fix) or "householder"."""
     import torch
     lam = spectrum(n, seed, kind)
-    Q = haar(n, seed, device)
+    Q = haar(n, seed, device) if basis == "haar" else householder(n, seed, device)
     L = torch.from_numpy(lam).to(device)
     X = (Q * L) @ Q.T
     X = (X + X.T) * 0.5
