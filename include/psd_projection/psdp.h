@@ -53,7 +53,7 @@ typedef enum psdp_status {
 typedef enum psdp_format {
   PSDP_FMT_F32 = 0,    /* float32 operands on the FP32 MFMA                       */
   PSDP_FMT_F16 = 1,    /* binary16 operands, float32 accumulation                  */
-  PSDP_FMT_F16X3 = 2,  /* hi + 2^-11 lo binary16 split, 3 products (~22-bit)       */
+  PSDP_FMT_F16X3 = 2,  /* 2^8 v = hi + lo binary16 split, 3 products (~22-bit)     */
   PSDP_FMT_BF16 = 3,   /* bfloat16 operands                                        */
   PSDP_FMT_BF16X3 = 4, /* hi + lo bfloat16 split, 3 products (~16-bit)             */
   PSDP_FMT_BF16X9 = 5  /* hi + mid + lo bfloat16 split, 9 products (~fp32)         */
