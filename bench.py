@@ -66,6 +66,104 @@ def measured_traffic(n, fmt):
         return None
 
 
+MIXED_UNITS = 13 * 3 + 9  # f16x3 products x 3 + f16 products, mixed schedule
+
+
+def bench_batched(args, P, W, torch):
+    """Config 4: 256 blocks, n log-uniform in [32, 1024] (seed 4), lambda ~ U[-1,1],
+    one grouped launch per filter-chain product for all blocks."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    sizes = [int(round(float(np.exp(rng.uniform(np.log(32), np.log(1024)))))) for _ in range(args.batched)]
+    xs, ps = [], []
+    for i, n in enumerate(sizes):
+        X64, Pm, _ = W.spectral_matrix(n, 4000 + i, "uniform")
+        xs.append(X64.float().contiguous())
+        ps.append(Pm)
+    h = P.Handle(0, seed=4)
+    cfg = P.default_config(args.method, synchronize=0)
+    outs = P.project_batched(h, xs, config=cfg)
+    torch.cuda.synchronize()
+    reps = 5
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        P.project_batched(h, xs, outs=outs, config=cfg)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    errs = [W.rel_fro_error(y, pm) for y, pm in zip(outs, ps)]
+    flops = MIXED_UNITS * sum(float(n) * n * (n + 1) for n in sizes) if args.method == "mixed" else None
+    res = {"workload": f"{len(sizes)} blocks, n log-uniform [32,1024] seed 4 (sum n = {sum(sizes)}), "
+                       f"lambda U[-1,1], {args.method}",
+           "ms_per_batch": round(ms, 3), "proj_per_s": round(len(sizes) / (ms * 1e-3), 1),
+           "max_rel_err": max(errs), "median_rel_err": float(np.median(errs))}
+    if flops:
+        res["mfma_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
+        res["mfma_frac"] = round(flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["f16"], 4)
+    log(f"batched: {res}")
+    return res
+
+
+def bench_served(args, P, W, torch):
+    """Config 5: Poisson stream of requests, n in {64,256,1024,4096,8192} with
+    weights {.4,.3,.15,.1,.05}, lambda ~ N(0,1), FP32 in/out; admission by HBM
+    budget, continuous batching over stream lanes.  Inputs come from a pool of
+    seeded matrices per size (request i uses pool entry i mod 4)."""
+    import numpy as np
+    from psd_projection_amd.serve import ProjectionServer, Request, TorchExecutor, latency_stats
+    sizes = [64, 256, 1024, 4096, 8192]
+    weights = [0.4, 0.3, 0.15, 0.1, 0.05]
+    pool = {}
+    for n in sizes:
+        pool[n] = []
+        for j in range(4):
+            X64, Pm, _ = W.spectral_matrix(n, 5000 + 10 * n + j, "normal")
+            pool[n].append((X64.float().contiguous(), Pm, Pm.norm()))
+            del X64
+    rng = np.random.default_rng(5)
+
+    def make(count):
+        ns = rng.choice(sizes, size=count, p=weights)
+        return [Request(i, int(n), pool[int(n)][i % 4][0], 0.0) for i, n in enumerate(ns)]
+
+    ex = TorchExecutor(0, args.method, small_lanes=2, large_lanes=2, seed=5)
+    ex.warm(sizes)
+    budget = 64 << 30
+    # capacity: a burst (everything arrives at once)
+    burst = make(min(1000, args.serve_requests))
+    srv = ProjectionServer(ex, budget)
+    t0 = time.perf_counter()
+    srv.run(burst, [0.0] * len(burst))
+    torch.cuda.synchronize()
+    cap = len(burst) / (time.perf_counter() - t0)
+    # timed Poisson stream at load x capacity
+    rate = args.serve_load * cap
+    reqs = make(args.serve_requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs))).tolist()
+    errs = []
+
+    def on_done(r):
+        X, Pm, pn = pool[r.n][r.rid % 4]
+        with torch.cuda.stream(ex.stream_of(r.lane)):
+            errs.append((r.Y.double() - Pm).norm() / pn)
+        r.Y = None
+
+    srv = ProjectionServer(ex, budget, on_done=on_done)
+    done = srv.run(reqs, arrivals)
+    torch.cuda.synchronize()
+    st = latency_stats(done)
+    e = torch.stack(errs).cpu().numpy() if errs else np.zeros(1)
+    res = {"workload": f"{len(reqs)} requests (full config: 10000), Poisson at {rate:.1f}/s = {args.serve_load} x "
+                       f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}",
+           "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+           "proj_per_s": round(st["proj_per_s"], 1), "capacity_proj_per_s": round(cap, 1),
+           "launches": srv.batches, "peak_admitted_bytes": srv.peak_used,
+           "max_rel_err": float(e.max()), "requests_over_1e-3": int((e > 1e-3).sum())}
+    log(f"served: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +173,10 @@ def main():
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=6144, help="CPU baseline block size (0: skip)")
+    ap.add_argument("--batched", type=int, default=256, help="config 4 block count (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="config 5 requests in the timed Poisson stream (0: skip; the full config is 10000)")
+    ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
     ap.add_argument("--basis", default="haar", choices=("haar", "householder"),
                     help="eigenvector basis of the synthetic input (householder: no QR, for counter runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -188,6 +290,13 @@ def main():
         err32 = W.rel_fro_error(Y, Pm)
         log(f"composite_FP32: {ms32:.2f} ms/projection, rel err {err32:.3e}")
         out["fp32_chain"] = {"ms": round(ms32, 3), "rel_err": err32, "speedup": round(ms32 / ms, 2)}
+
+    del X, Y, Pm
+    torch.cuda.empty_cache()
+    if args.batched > 0:
+        out["batched"] = bench_batched(args, P, W, torch)
+    if args.serve_requests > 0:
+        out["served"] = bench_served(args, P, W, torch)
 
     if cpu_block is not None:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
