@@ -66,7 +66,7 @@ def measured_traffic(n, fmt):
         return None
 
 
-MIXED_UNITS = 13 * 3 + 9  # f16x3 products x 3 + f16 products, mixed schedule
+MIXED_UNITS = 12 * 3 + 10  # mixed schedule: 12 FP16x3 products (3 MFMA products each) + 10 binary16
 
 
 def bench_batched(args, P, W, torch):
@@ -251,8 +251,9 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[fmt],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
                 "traffic": measured_traffic(n, fmt),
-                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel)", "launches": cnt,
-                "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
+                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel: 256x256 tiles in whole "
+                          "rounds + the last partial round as 128x128 tiles in a second launch)",
+                "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
                 "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
     total_chain_flops = sum(PRODUCTS[f] * wsym * c for f, (c, _) in prods.items()) / args.steps
     chain_ms = sum(t for _, t in prods.values()) / args.steps
@@ -262,8 +263,8 @@ def main():
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f16" if args.method in ("mixed", "composite_FP16") else
            ("f32" if args.method == "composite_FP32" else args.method),
-           "dtype_note": "binary16 MFMA operands with f32 accumulation; steps 0-3 and the final product "
-                         "use a 3-term hi/lo binary16 split (FP32-level products); Lanczos in f64"
+           "dtype_note": "binary16 MFMA operands with f32 accumulation; steps 0-3 use a 3-term hi/lo "
+                         "binary16 split (FP32-level products); Lanczos in f64"
            if args.method == "mixed" else "",
            "data": ("synthetic: Haar Q (QR of a seeded Gaussian, torch FP64 on the GPU)" if args.basis == "haar"
                     else "synthetic: Householder Q (one seeded reflector)") +

@@ -71,7 +71,7 @@ typedef enum psdp_method {
   PSDP_METHOD_COMPOSITE_FP32 = 0,          /* FP32 table, FP32 MFMA everywhere          */
   PSDP_METHOD_COMPOSITE_FP16 = 1,          /* FP16 table, FP16 operands everywhere      */
   PSDP_METHOD_COMPOSITE_FP32_EMULATED = 2, /* FP32 table, BF16x9 everywhere             */
-  PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 steps 0-3 + final      */
+  PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 on steps 0-3, FP16 rest */
   PSDP_METHOD_CUSTOM = 4                   /* table + formats taken from psdp_config_t  */
 } psdp_method_t;
 

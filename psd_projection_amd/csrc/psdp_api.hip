@@ -155,12 +155,16 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
       s.final_fmt = FMT_BF16X9;
       break;
     case PSDP_METHOD_MIXED:
-      // FP16-table chain: hi/lo split operands on the first 4 steps, where
-      // rounding errors are amplified by the later steps, plain binary16 on
-      // the last 3, split operands again for the final product.
+      // FP16-table chain: hi/lo split operands on steps 0-3, where rounding
+      // errors are amplified by the later steps, plain binary16 on steps 4-6
+      // and the final product, which only polish eigenvalues already near
+      // +-1.  Measured (n = 640 .. 16384, three spectra): split products on
+      // the final product change the error by < 1.2x; dropping them from any
+      // of steps 1-3 costs 2-10x, from step 0 up to 3x when the scaled
+      // spectrum reaches +-1.
       fill_table(PSDP_TABLE_FP16, s);
       for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 4 ? FMT_F16X3 : FMT_F16;
-      s.final_fmt = FMT_F16X3;
+      s.final_fmt = FMT_F16;
       break;
     case PSDP_METHOD_CUSTOM:
       if (cfg->table != PSDP_TABLE_FP32 && cfg->table != PSDP_TABLE_FP16)
@@ -210,6 +214,8 @@ struct psdp_handle_st {
   uint64_t seed = 0x5eed5eedULL;
   void* blas = nullptr;  // rocBLAS handle for the EVD drop-ins (created on first use)
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
+  std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, 128-tile tail)
+  int cus = 256;                                  // compute units of the device
   std::mutex mu;
 };
 
@@ -217,7 +223,7 @@ struct psdp_handle_st {
 // tiles running together share operand panels, then interleaved so that the
 // hardware's round-robin dispatch gives each of the 8 XCDs one contiguous
 // chunk of that order (its L2 then serves neighbouring panels).
-static std::vector<TileRef> make_tile_order(int T) {
+static std::vector<TileRef> super_tile_order(int T) {
   const int G = 4;
   std::vector<TileRef> ord;
   ord.reserve((size_t)T * (T + 1) / 2);
@@ -225,6 +231,10 @@ static std::vector<TileRef> make_tile_order(int T) {
     for (int sj = si; sj < T; sj += G)
       for (int i = si; i < std::min(si + G, T); ++i)
         for (int j = std::max(sj, i); j < std::min(sj + G, T); ++j) ord.push_back(TileRef{0, i, j, 0});
+  return ord;
+}
+
+static std::vector<TileRef> xcd_interleave(const std::vector<TileRef>& ord) {
   const int L = (int)ord.size();
   const int XCD = 8;
   const int C = (L + XCD - 1) / XCD;
@@ -236,6 +246,14 @@ static std::vector<TileRef> make_tile_order(int T) {
   return out;
 }
 
+static psdp_status_t upload_tiles(const std::vector<TileRef>& v, TileList& n) {
+  n.count = (int)v.size();
+  if (hipMalloc(&n.dev, std::max<size_t>(v.size(), 1) * sizeof(TileRef)) != hipSuccess)
+    return fail(PSDP_ERR_NOMEM, "tile list allocation failed");
+  if (!v.empty()) HIPCHK(hipMemcpy(n.dev, v.data(), v.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  return PSDP_OK;
+}
+
 static psdp_status_t get_tiles(psdp_handle_t h, int T, int bt, TileList& tl) {
   auto key = std::make_pair(T, bt);
   auto it = h->tiles.find(key);
@@ -243,14 +261,49 @@ static psdp_status_t get_tiles(psdp_handle_t h, int T, int bt, TileList& tl) {
     tl = it->second;
     return PSDP_OK;
   }
-  std::vector<TileRef> v = make_tile_order(T);
   TileList n;
-  n.count = (int)v.size();
-  if (hipMalloc(&n.dev, v.size() * sizeof(TileRef)) != hipSuccess)
-    return fail(PSDP_ERR_NOMEM, "tile list allocation failed");
-  HIPCHK(hipMemcpy(n.dev, v.data(), v.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  STCHK(upload_tiles(xcd_interleave(super_tile_order(T)), n));
   h->tiles[key] = n;
   tl = n;
+  return PSDP_OK;
+}
+
+// 256 x 256 tiles in whole rounds over the device's CUs, plus the last,
+// partial round re-cut into 128 x 128 tiles for a second launch: one tile per
+// CU at a time, so a partial round of big tiles (2080 tiles = 8.1 rounds at
+// n = 16384 on 256 CUs) would cost a whole round; as small tiles it costs
+// about a quarter.  Only when the partial round is at most half the CUs.
+static psdp_status_t get_split_tiles(psdp_handle_t h, int T, TileList& big, TileList& small) {
+  auto it = h->split_tiles.find(T);
+  if (it != h->split_tiles.end()) {
+    big = it->second.first;
+    small = it->second.second;
+    return PSDP_OK;
+  }
+  const std::vector<TileRef> ord = super_tile_order(T);
+  const int total = (int)ord.size(), ncu = std::max(h->cus, 1);
+  const int full = total / ncu * ncu, rem = total - full;
+  std::vector<TileRef> bigv, smallv;
+  if (full == 0 || rem == 0 || 2 * rem > ncu) {
+    bigv = xcd_interleave(ord);
+  } else {
+    bigv = xcd_interleave(std::vector<TileRef>(ord.begin(), ord.begin() + full));
+    std::vector<TileRef> tail;
+    for (int t = full; t < total; ++t)
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          const int r = 2 * ord[t].bi + a, c = 2 * ord[t].bj + b;
+          if (r <= c) tail.push_back(TileRef{0, r, c, 0});  // the lower quarter of a diagonal tile is a mirror
+        }
+    smallv = xcd_interleave(tail);
+  }
+  TileList nb, ns;
+  STCHK(upload_tiles(bigv, nb));
+  STCHK(upload_tiles(smallv, ns));
+  if (smallv.empty()) ns.count = 0;
+  h->split_tiles[T] = std::make_pair(nb, ns);
+  big = nb;
+  small = ns;
   return PSDP_OK;
 }
 
@@ -462,9 +515,16 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   auto product = [&](int fmt, int avg, const ProdDesc& d) -> psdp_status_t {
     DESCCHK(d, fmt);
     const int sub = launch_sub(p.n, fmt, avg);
+    ProfScope ps(h, PSDP_PROF_PROD_F32 + fmt);
+    if (sub == 8) {
+      TileList big, small;
+      STCHK(get_split_tiles(h, p.npad / 256, big, small));
+      HIPCHK(launch_sym_product(fmt, 8, avg, d, big.dev, big.count, s));
+      if (small.count) HIPCHK(launch_sym_product(fmt, 4, avg, d, small.dev, small.count, s));
+      return PSDP_OK;
+    }
     TileList tl;
     STCHK(get_tiles(h, p.npad / (32 * sub), 32 * sub, tl));
-    ProfScope ps(h, PSDP_PROF_PROD_F32 + fmt);
     HIPCHK(launch_sym_product(fmt, sub, avg, d, tl.dev, tl.count, s));
     return PSDP_OK;
   };
@@ -625,6 +685,9 @@ psdp_status_t psdp_create(psdp_handle_t* out, int device) {
     return fail(PSDP_ERR_DEVICE, "stream creation failed");
   }
   h->own_stream = true;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    h->cus = cus;
   if (hipMalloc(&h->st, sizeof(LanczosState)) != hipSuccess) {
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -668,6 +731,10 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   (void)hipStreamSynchronize(h->stream);
   clear_profile(h);
   for (auto& kv : h->tiles) (void)hipFree(kv.second.dev);
+  for (auto& kv : h->split_tiles) {
+    (void)hipFree(kv.second.first.dev);
+    (void)hipFree(kv.second.second.dev);
+  }
   if (h->arena) (void)hipFree(h->arena);
   if (h->st) (void)hipFree(h->st);
   eig_destroy_blas(h->blas);
@@ -827,8 +894,9 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
     py[i] = (uint16_t*)(c + 2 * f32b + (3 + i) * u16b);
   }
   hipStream_t s = h->stream;
-  TileList tl;
-  STCHK(get_tiles(h, npad / bt, bt, tl));
+  TileList tl, tail;
+  if (sub == 8) STCHK(get_split_tiles(h, npad / bt, tl, tail));  // the tiling the projections use
+  else STCHK(get_tiles(h, npad / bt, bt, tl));
   {
     ProfScope ps(h, PSDP_PROF_PREP);
     HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, ld, npad, s));
@@ -852,6 +920,7 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   {
     ProfScope ps(h, PSDP_PROF_PROD_F32 + format);
     HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
+    if (tail.count) HIPCHK(launch_sym_product(format, 4, average ? 1 : 0, d, tail.dev, tail.count, s));
   }
   HIPCHK(hipStreamSynchronize(s));
   return PSDP_OK;

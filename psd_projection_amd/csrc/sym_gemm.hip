@@ -138,8 +138,11 @@ __device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int
 // Tiles on the diagonal write only i <= j first and i < j in the mirror; a
 // mirror store never lands on an element another phase reads (it is below
 // the diagonal), so the phases of one tile do not interfere.
-template <int BT>
-__device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR) {
+// cscale multiplies the staged values (the 2^-16 of FP16x3 products when the
+// tile stored raw accumulators).
+template <int BT, int NT = 256>
+__device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
+                                              float cscale = 1.0f) {
   constexpr int CS = BT + 1;
   constexpr int Q4 = BT / 4;
   const int tid = threadIdx.x;
@@ -148,10 +151,10 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const int64_t ld = d.ld;
   const int nq = NR / 4;
   if (d.mode == EPI_OUTPUT) {
-    const double s = d.scale ? *d.scale : 1.0;
+    const double s = (d.scale ? *d.scale : 1.0) * (double)cscale;
     const int64_t old = d.out_ld;
     // (i, j) at i + j*ld: contiguous along i
-    for (int q = tid; q < BT * nq; q += 256) {
+    for (int q = tid; q < BT * nq; q += NT) {
       const int c = q / nq, rr = (q % nq) * 4;
       const int j = col0 + c;
 #pragma unroll
@@ -165,7 +168,7 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
       }
     }
     // (j, i) at j + i*ld: contiguous along j
-    for (int q = tid; q < NR * Q4; q += 256) {
+    for (int q = tid; q < NR * Q4; q += NT) {
       const int r = q / Q4, c0 = (q % Q4) * 4;
       const int i = row0 + r;
 #pragma unroll
@@ -184,13 +187,13 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const bool half_id = d.mode == EPI_POLY_HALF_ID;
   const bool pieces = d.out_fmt != FMT_F32;
   // pass 1: primary block, rows of the tile
-  for (int q = tid; q < NR * Q4; q += 256) {
+  for (int q = tid; q < NR * Q4; q += NT) {
     const int r = q / Q4, c0 = (q % Q4) * 4;
     const int i = row0 + r, j0 = col0 + c0;
     const int64_t idx = (int64_t)i * ld + j0;
     float w[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e];
+    for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
     if (poly) {
       const f32x4 a4 = *(const f32x4*)(d.c32 + idx);
       const f32x4 p4 = *(const f32x4*)(d.p3 + idx);
@@ -215,14 +218,14 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
         }
       }
     }
-    if (poly) {
+    if (poly || cscale != 1.0f) {  // pass 2 reads final values
 #pragma unroll
       for (int e = 0; e < 4; ++e) C[r * CS + c0 + e] = w[e];
     }
   }
   __syncthreads();
   // pass 2: mirror block, columns of the tile
-  for (int q = tid; q < BT * nq; q += 256) {
+  for (int q = tid; q < BT * nq; q += NT) {
     const int c = q / nq, rr = (q % nq) * 4;
     const int j = col0 + c, i0 = row0 + rr;
     const int64_t idx = (int64_t)j * ld + i0;
@@ -437,7 +440,7 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
 // 128-byte row segments with consecutive lanes, as coalescing wants.
 // The loop waits for its own loads with an explicit vmcnt and crosses a bare
 // s_barrier (a __syncthreads would also wait for the slabs still in flight).
-template <int FMT, int SUB>
+template <int FMT, int SUB, bool WS = false>
 struct DmaTiling {
   static constexpr int P = FmtTraits<FMT>::P;
   static constexpr int WT = 16 * SUB;
@@ -445,9 +448,11 @@ struct DmaTiling {
   // Small tiles keep LDS under 80 KB so that two workgroups share a CU.
   static constexpr int LDS_BUDGET = SUB <= 2 ? 81920 : 163840;
   // 64-element slabs make every row segment a whole 128-byte line (half as
-  // many L2 requests as 32-element slabs); three-piece operands would not
-  // fit two such stages.
-  static constexpr int BK = P == 3 ? 32 : 64;              // slab width (elements)
+  // many L2 requests as 32-element slabs) where two such stages fit in LDS;
+  // with producer waves two-piece operands also take 32-element slabs (the
+  // fragments of 64 would not fit 256 registers per wave).
+  static constexpr bool FIT64 = 2 * (2 * P * BT * 128) <= LDS_BUDGET;
+  static constexpr int BK = (FIT64 && !(WS && P == 2)) ? 64 : 32;  // slab width (elements)
   static constexpr int RB = BK * 2;                        // row bytes
   static constexpr int TILE_BYTES = BT * RB;               // one (panel, piece)
   static constexpr int STAGE_BYTES = 2 * P * TILE_BYTES;   // X rows of bi, Y rows of bj
@@ -488,14 +493,24 @@ __device__ __forceinline__ void lds_barrier() {
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-template <int FMT, int SUB>
+// WS (producer waves): the workgroup has 8 waves; waves 0-3 compute (one per
+// SIMD) and never issue a global->LDS load, waves 4-7 only issue them and
+// wait for them.  A wave that issues a 1 KB global_load_lds stalls its own
+// instruction stream for many cycles, and with one compute wave per SIMD
+// those cycles are lost to the matrix cores; with a producer wave beside it
+// the compute wave keeps issuing MFMAs.  Requires <= 256 registers per wave.
+template <int FMT, int SUB, bool WS>
 __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, char* smem) {
-  using DT = DmaTiling<FMT, SUB>;
+  using DT = DmaTiling<FMT, SUB, WS>;
   constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, S = DT::S, L = DT::L;
   constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
+  constexpr int NT = WS ? 512 : 256;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (SGPR)
-  const int wr = wave >> 1, wc = wave & 1;
+  const bool compute = !WS || wave < 4;                         // issues MFMAs
+  const bool loader = !WS || wave >= 4;                         // issues global->LDS loads
+  const int dwave = WS ? wave - 4 : wave;                       // index among the loaders
+  const int wr = (wave & 3) >> 1, wc = wave & 1;
   const int64_t ld = d.ld;
   const int row0 = bi * DT::BT, col0 = bj * DT::BT;
 
@@ -508,7 +523,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   int lofs[L], loff[L];
 #pragma unroll
   for (int i = 0; i < L; ++i) {
-    const int q = wave * L + i;
+    const int q = dwave * L + i;
     const int gp = q / DT::BLOCKS, blk = q % DT::BLOCKS;
     const int g = gp / P, p = gp % P;
     // select chain rather than a dynamic index: keeps a descriptor held in
@@ -561,20 +576,24 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   // Every iteration issues one stage (past the end: a copy of the last slab
   // into a buffer that is not read again), so that the wait counts are
   // uniform and the loop body is one block that the scheduler can interleave.
+  if (loader) {
 #pragma unroll
-  for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((st < KT ? st : KT - 1) * BK, st) }
-  wait_vmcnt<(S - 1) * L>();
+    for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((st < KT ? st : KT - 1) * BK, st) }
+    wait_vmcnt<(S - 1) * L>();
+  }
   lds_barrier();
+  if (compute) {
 #pragma unroll
-  for (int ks = 0; ks < KS; ++ks)
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-    for (int p = 0; p < P; ++p)
+      for (int p = 0; p < P; ++p)
 #pragma unroll
-      for (int m = 0; m < M; ++m) {
-        fa[ks][p][m] = frag_a(smem, ks, p, m);
-        fb[ks][p][m] = frag_b(smem, ks, p, m);
-      }
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        for (int m = 0; m < M; ++m) {
+          fa[ks][p][m] = frag_a(smem, ks, p, m);
+          fb[ks][p][m] = frag_b(smem, ks, p, m);
+        }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  }
 
   // MFMAs of column block mj of k-step ks
 #define PSDP_COLUMN_MMA(ks, mj)                                                                   \
@@ -601,10 +620,11 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   constexpr int MR = NMFMA / NREAD > 0 ? NMFMA / NREAD : 1;
   for (int kt = 0; kt + 1 < KT; ++kt) {
     // own reads of buffer kt % S complete, own DMA of slab kt + 1 landed
-    wait_vmcnt<(S - 2) * L>();
+    if (loader) wait_vmcnt<(S - 2) * L>();
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     lds_barrier();
-    PSDP_DMA_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S)
+    if (loader) { PSDP_DMA_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S) }
+    if (!compute) continue;
     const char* nxt = smem + ((kt + 1) % S) * SB;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
@@ -625,7 +645,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     // stalls the wave on the LDS command queue with the matrix cores idle).
 #pragma unroll
     for (int g = 0; g < NREAD; ++g) {
-      if (g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+      if (!WS && g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
       __builtin_amdgcn_sched_group_barrier(0x008, MR, 0);  // MFMA
     }
@@ -637,13 +657,15 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
 #pragma unroll
         for (int m = 0; m < M; ++m) fa[ks][p][m] = na[ks][p][m];
   }
+  if (compute) {
 #pragma unroll
-  for (int ks = 0; ks < KS; ++ks)
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-    for (int mj = 0; mj < M; ++mj) { PSDP_COLUMN_MMA(ks, mj) }
+      for (int mj = 0; mj < M; ++mj) { PSDP_COLUMN_MMA(ks, mj) }
+  }
 #undef PSDP_COLUMN_MMA
   __builtin_amdgcn_sched_barrier(0);  // keep the epilogue's accumulator reads after the MFMAs
-  wait_vmcnt<0>();  // trailing stage copies land before the epilogue reuses LDS
+  if (loader) wait_vmcnt<0>();  // trailing stage copies land before the epilogue reuses LDS
 #undef PSDP_DMA_ISSUE
   lds_barrier();  // every wave is done reading the stages before they hold C
 
@@ -654,7 +676,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   constexpr int PHASES = DT::BT / DT::HR;
 #pragma unroll
   for (int ph = 0; ph < PHASES; ++ph) {
-    if (PHASES == 1 || wr == ph) {
+    if (compute && (PHASES == 1 || wr == ph)) {
 #pragma unroll
       for (int mi = 0; mi < M; ++mi)
 #pragma unroll
@@ -663,21 +685,21 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
           for (int e = 0; e < 16; ++e) {
             const int r = wr * DT::WT + mi * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3) - ph * DT::HR;
             const int c = wc * DT::WT + mj * 32 + (lane & 31);
-            Ct[r * CS + c] = acc[mi][mj][e] * PSCALE;
+            Ct[r * CS + c] = acc[mi][mj][e];
           }
     }
     __syncthreads();
-    epilogue_rows<DT::BT>(d, bi, bj, Ct, ph * DT::HR, DT::HR);
+    epilogue_rows<DT::BT, NT>(d, bi, bj, Ct, ph * DT::HR, DT::HR, PSCALE);
     __syncthreads();
   }
 }
 
-template <int FMT, int SUB>
-__global__ __launch_bounds__(256) void sym_prod_dma_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
-  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB>::LDS_BYTES];
+template <int FMT, int SUB, bool WS>
+__global__ __launch_bounds__(WS ? 512 : 256) void sym_prod_dma_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB, WS>::LDS_BYTES];
   const TileRef t = tiles[blockIdx.x];
   if (t.bi < 0) return;
-  sym_tile_dma<FMT, SUB>(d, t.bi, t.bj, smem);
+  sym_tile_dma<FMT, SUB, WS>(d, t.bi, t.bj, smem);
 }
 
 template <int FMT, int SUB>
@@ -686,7 +708,7 @@ __global__ __launch_bounds__(256) void sym_prod_dma_grouped_kernel(const ProdDes
   __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB>::LDS_BYTES];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
-    sym_tile_dma<FMT, SUB>(descs[tr.mat], tr.bi, tr.bj, smem);  // descriptor read in place
+    sym_tile_dma<FMT, SUB, false>(descs[tr.mat], tr.bi, tr.bj, smem);  // descriptor read in place
   }
 }
 
@@ -729,13 +751,19 @@ static hipError_t launch_grouped(const ProdDesc* descs, const TileRef* tiles, in
   return hipGetLastError();
 }
 
-// 256 x 256 tiles (SUB = 8) exist for the 1-piece 16-bit formats with plain
-// (non-averaged) products.
-bool sym_big_tile_capable(int fmt, int avg) { return !avg && (fmt == FMT_F16 || fmt == FMT_BF16); }
+// 256 x 256 tiles (SUB = 8) exist for the 1- and 2-piece 16-bit formats with
+// plain (non-averaged) products.
+bool sym_big_tile_capable(int fmt, int avg) {
+  return !avg && (fmt == FMT_F16 || fmt == FMT_BF16 || fmt == FMT_F16X3 || fmt == FMT_BF16X3);
+}
 
+// Producer waves (WS) are kept available but off: measured at n = 16384 they
+// ran the 128 x 128 FP16x3 tiles 5% slower than the plain kernel, whose
+// 64-element slabs do not fit the 256 registers a second wave per SIMD leaves.
 template <int FMT, int SUB>
 static hipError_t launch_dma(const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
-  hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, SUB>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+  constexpr bool WS = false;
+  hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, SUB, WS>), dim3(ntiles), dim3(WS ? 512 : 256), 0, s, d, tiles);
   return hipGetLastError();
 }
 
@@ -770,20 +798,26 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
   case F:                                                                       \
     return sub == 4 ? launch_dma<F, 4>(d, tiles, ntiles, s)                     \
                     : (sub == 2 ? launch_dma<F, 2>(d, tiles, ntiles, s) : hipErrorInvalidValue);
+#define PSDP_CASE2(F)                                                           \
+  case F:                                                                       \
+    return sub == 8 ? launch_dma<F, 8>(d, tiles, ntiles, s)                     \
+                    : (sub == 4 ? launch_dma<F, 4>(d, tiles, ntiles, s)         \
+                                : (sub == 2 ? launch_dma<F, 2>(d, tiles, ntiles, s) : hipErrorInvalidValue));
   switch (fmt) {
     case FMT_F32:
       return sub == 4 ? launch_one<FMT_F32, 4, false>(d, tiles, ntiles, s)
                       : (sub == 2 ? launch_one<FMT_F32, 2, false>(d, tiles, ntiles, s) : hipErrorInvalidValue);
     PSDP_CASE1(FMT_F16)
     PSDP_CASE1(FMT_BF16)
-    PSDP_CASEN(FMT_F16X3)
-    PSDP_CASEN(FMT_BF16X3)
+    PSDP_CASE2(FMT_F16X3)
+    PSDP_CASE2(FMT_BF16X3)
     PSDP_CASEN(FMT_BF16X9)
     default:
       return hipErrorInvalidValue;
   }
 #undef PSDP_CASE1
 #undef PSDP_CASEN
+#undef PSDP_CASE2
 }
 
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,

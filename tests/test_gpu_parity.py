@@ -73,6 +73,22 @@ def test_sym_product(handle, fmt, avg, n):
     assert err <= 2e-6, err
 
 
+@pytest.mark.parametrize("fmt", ["f16", "f16x3"])
+def test_sym_product_split_tail(handle, fmt):
+    # n = 6144: 300 tiles of 256 x 256, i.e. one round on 256 CUs plus a partial
+    # one that runs as 128 x 128 tiles in a second launch (on devices with
+    # fewer CUs the split differs; the result must not).
+    n = 6144
+    print(f"sym_product split tail fmt={fmt} n={n}", flush=True)
+    X, _, _ = O.spectral_input(n, 77, "uniform")
+    X32 = X.astype(np.float32)
+    C = from_dev(P.sym_product(handle, to_dev(X32, torch.float32), to_dev(X32, torch.float32), fmt))
+    ref = O.sym_product(X32, X32, fmt, "mirror").astype(np.float64)
+    assert np.array_equal(C, C.T)
+    err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
+    assert err <= 2e-6, err
+
+
 def test_sym_product_average_rejected_for_split_formats(handle):
     X = torch.eye(8, device=DEV)
     with pytest.raises(P.PsdpError):
@@ -151,7 +167,7 @@ def oracle_error(X, P_true, method, seed):
     elif method == "composite_FP32_emulated":
         Y = O.composite_chain(Xs, c32, ["bf16x9"] * 10, "bf16x9", sym_mode="mirror")
     elif method == "mixed":
-        Y = O.composite_chain(Xs, c16, ["f16x3"] * 4 + ["f16"] * 3, "f16x3", sym_mode="mirror")
+        Y = O.composite_chain(Xs, c16, ["f16x3"] * 4 + ["f16"] * 3, "f16", sym_mode="mirror")
     return O.rel_fro_error(Y * up, P_true)
 
 
