@@ -9,7 +9,7 @@
 //     iteration, see step_body); after a break the remaining launches are
 //     no-ops;
 //   * T = tridiag(alpha, beta) (src/lanczos.cu:16-38), its largest eigenpair
-//     by a one-wave cyclic Jacobi solver in FP64 (replaces cusolverDnDsyevd
+//     by multisection + inverse iteration in FP64 (replaces cusolverDnDsyevd
 //     at :206-214; T is built from zeros, unlike the unzeroed buffer there);
 //   * y = V u (:242), r = A y - theta y (the reference's :255 copies ry into
 //     w1, so its residual is A y - theta y), lo = sqrt(theta) (:264),
@@ -182,11 +182,30 @@ __device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int m, 
   }
 }
 
-// Largest eigenpair of T = tridiag(alpha[0..nb), beta[0..nb-1)) by cyclic
-// Jacobi rotations in FP64, one wave (lane k owns row/column k).
+// Largest eigenpair of T = tridiag(alpha[0..nb), beta[0..nb-1)) in FP64,
+// one wave (replaces cusolverDnDsyevd on T, src/lanczos.cu:206-214):
+//   * the eigenvalue by multisection on Sturm counts: the 64 lanes evaluate
+//     the number of eigenvalues below 64 points of the bracket at once, so
+//     each round shrinks it 65x (about 9 rounds to full precision);
+//   * the eigenvector by inverse iteration on T - theta I with a partially
+//     pivoted tridiagonal LU (the LAPACK dgttrf/dgtts2 scheme), 3 solves.
+__device__ __forceinline__ int sturm_count(const double* a, const double* b2, int nb, double x, double pivmin) {
+  int cnt = 0;
+  double d = 1.0;
+  for (int i = 0; i < nb; ++i) {
+    d = (a[i] - x) - (i ? b2[i - 1] / d : 0.0);
+    if (fabs(d) < pivmin) d = -pivmin;
+    cnt += d < 0.0;
+  }
+  return cnt;  // eigenvalues below x
+}
+
 __device__ void ritz_body(LanczosState* st) {
-  __shared__ double T[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
-  __shared__ double Z[LANCZOS_MAX_ITER][LANCZOS_MAX_ITER + 1];
+  __shared__ double a[LANCZOS_MAX_ITER], b[LANCZOS_MAX_ITER], b2[LANCZOS_MAX_ITER];
+  // work arrays of the (serial) inverse iteration, in LDS
+  __shared__ double u[LANCZOS_MAX_ITER], dl[LANCZOS_MAX_ITER], dd[LANCZOS_MAX_ITER], du[LANCZOS_MAX_ITER],
+      du2[LANCZOS_MAX_ITER];
+  __shared__ int piv[LANCZOS_MAX_ITER];
   const int lane = threadIdx.x;
   const int nb = st->nb;
   if (nb == 0 || st->nonfinite) {
@@ -204,87 +223,97 @@ __device__ void ritz_body(LanczosState* st) {
     }
     return;
   }
-  for (int c = 0; c < nb; ++c) {
-    if (lane < nb) {
-      double t = 0.0;
-      if (lane == c) t = st->alpha[c];
-      else if (lane == c + 1) t = st->beta[c];
-      else if (lane + 1 == c) t = st->beta[lane];
-      T[lane][c] = t;
-      Z[lane][c] = lane == c ? 1.0 : 0.0;
-    }
+  if (lane < nb) {
+    a[lane] = st->alpha[lane];
+    b[lane] = lane + 1 < nb ? st->beta[lane] : 0.0;
+    b2[lane] = b[lane] * b[lane];
   }
   __syncthreads();
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, dia = 0.0;
-    if (lane < nb) {
-      for (int c = 0; c < nb; ++c) {
-        const double t = T[lane][c];
-        if (c == lane) dia += t * t;
-        else off += t * t;
-      }
-    }
-    off = wave_sum(off);
-    dia = wave_sum(dia);
-    if (off <= 1e-30 * dia || off == 0.0) break;
-    for (int p = 0; p < nb - 1; ++p) {
-      for (int q = p + 1; q < nb; ++q) {
-        const double apq = T[p][q];
-        const double app = T[p][p], aqq = T[q][q];
-        if (fabs(apq) <= 1e-300 || fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq))) {
-          __syncthreads();
-          if (lane == 0) {
-            T[p][q] = 0.0;
-            T[q][p] = 0.0;
-          }
-          __syncthreads();
-          continue;
-        }
-        const double th = (aqq - app) / (2.0 * apq);
-        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0);
-        const double s = t * c;
-        double kp = 0.0, kq = 0.0, zp = 0.0, zq = 0.0;
-        if (lane < nb) {
-          kp = T[lane][p];
-          kq = T[lane][q];
-          zp = Z[lane][p];
-          zq = Z[lane][q];
-        }
-        __syncthreads();
-        if (lane < nb) {
-          if (lane != p && lane != q) {
-            const double np_ = c * kp - s * kq;
-            const double nq_ = s * kp + c * kq;
-            T[lane][p] = np_;
-            T[p][lane] = np_;
-            T[lane][q] = nq_;
-            T[q][lane] = nq_;
-          }
-          Z[lane][p] = c * zp - s * zq;
-          Z[lane][q] = s * zp + c * zq;
-        }
-        if (lane == 0) {
-          T[p][p] = app - t * apq;
-          T[q][q] = aqq + t * apq;
-          T[p][q] = 0.0;
-          T[q][p] = 0.0;
-        }
-        __syncthreads();
-      }
+  // Gershgorin bracket and the pivot floor of the Sturm recurrence
+  double glo = 1e308, ghi = -1e308, bmax = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const double r = fabs(b[i]) + (i ? fabs(b[i - 1]) : 0.0);
+    glo = fmin(glo, a[i] - r);
+    ghi = fmax(ghi, a[i] + r);
+    bmax = fmax(bmax, b2[i]);
+  }
+  const double pivmin = 1e-290 * fmax(1.0, bmax);
+  const double wid = fmax(ghi - glo, 1e-300);
+  double L = glo - 1e-12 * wid - 1e-300, H = ghi + 1e-12 * wid + 1e-300;  // count(L) = 0, count(H) = nb
+  for (int round = 0; round < 64; ++round) {
+    if (H - L <= 4.0 * 2.2204460492503131e-16 * fmax(fabs(L), fabs(H)) || H - L <= 1e-300) break;
+    const double x = L + (H - L) * (double)(lane + 1) / 65.0;
+    const bool top = sturm_count(a, b2, nb, x, pivmin) == nb;  // every eigenvalue below x
+    const unsigned long long m = __ballot(top);
+    if (m) {
+      const int j = __ffsll((long long)m) - 1;                // first point above the largest
+      const double xj = L + (H - L) * (double)(j + 1) / 65.0;
+      const double xl = j ? L + (H - L) * (double)j / 65.0 : L;
+      H = xj;
+      L = xl;
+    } else {
+      L = L + (H - L) * 64.0 / 65.0;
     }
   }
-  double best = -1e308;
-  int bi = 0;
-  for (int c = 0; c < nb; ++c) {
-    const double d = T[c][c];
-    if (d > best) {
-      best = d;
-      bi = c;
+  const double theta = 0.5 * (L + H);
+  if (lane == 0) {
+    const double tiny = 2.2204460492503131e-16 * fmax(fabs(glo), fabs(ghi)) + 1e-300;
+    // LU of T - theta I with partial pivoting (rows i, i+1)
+    for (int i = 0; i < nb; ++i) {
+      dd[i] = a[i] - theta;
+      dl[i] = b[i];
+      du[i] = b[i];
+      du2[i] = 0.0;
     }
+    for (int i = 0; i + 1 < nb; ++i) {
+      if (fabs(dd[i]) >= fabs(dl[i])) {
+        if (dd[i] == 0.0) dd[i] = tiny;
+        const double f = dl[i] / dd[i];
+        dl[i] = f;
+        dd[i + 1] -= f * du[i];
+        piv[i] = i;
+      } else {
+        const double f = dd[i] / dl[i];
+        dd[i] = dl[i];
+        dl[i] = f;
+        const double t = du[i];
+        du[i] = dd[i + 1];
+        dd[i + 1] = t - f * dd[i + 1];
+        if (i + 2 < nb) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+        piv[i] = i + 1;
+      }
+    }
+    if (fabs(dd[nb - 1]) < tiny) dd[nb - 1] = tiny;
+    for (int i = 0; i < nb; ++i) u[i] = 1.0;
+    for (int it = 0; it < 3; ++it) {
+      for (int i = 0; i + 1 < nb; ++i) {  // L solve
+        if (piv[i] == i) {
+          u[i + 1] -= dl[i] * u[i];
+        } else {
+          const double t = u[i];
+          u[i] = u[i + 1];
+          u[i + 1] = t - dl[i] * u[i];
+        }
+      }
+      for (int i = nb - 1; i >= 0; --i) {  // U solve
+        double r = u[i];
+        if (i + 1 < nb) r -= du[i] * u[i + 1];
+        if (i + 2 < nb) r -= du2[i] * u[i + 2];
+        u[i] = r / dd[i];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < nb; ++i) nrm = fmax(nrm, fabs(u[i]));
+      for (int i = 0; i < nb; ++i) u[i] /= nrm;
+    }
+    double s2 = 0.0;
+    for (int i = 0; i < nb; ++i) s2 += u[i] * u[i];
+    const double inv = 1.0 / sqrt(s2);
+    for (int i = 0; i < nb; ++i) st->u[i] = u[i] * inv;
+    st->theta = theta;
   }
-  if (lane < nb) st->u[lane] = Z[lane][bi];
-  if (lane == 0) st->theta = best;
 }
 
 // y = V(:, 0:nb) u  (src/lanczos.cu:242)

@@ -216,8 +216,41 @@ struct psdp_handle_st {
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
   std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, 128-tile tail)
   int cus = 256;                                  // compute units of the device
+  // pinned host staging ring for batched descriptors: slot k is reused only
+  // after the copy recorded in ev[k] has run
+  static constexpr int NSTAGE = 4;
+  char* stage[NSTAGE] = {};
+  size_t stage_bytes[NSTAGE] = {};
+  hipEvent_t stage_ev[NSTAGE] = {};
+  int stage_next = 0;
   std::mutex mu;
 };
+
+static psdp_status_t staging_slot(psdp_handle_t h, size_t bytes, char** out) {
+  const int k = h->stage_next;
+  if (h->stage_ev[k]) HIPCHK(hipEventSynchronize(h->stage_ev[k]));
+  if (h->stage_bytes[k] < bytes) {
+    if (h->stage[k]) HIPCHK(hipHostFree(h->stage[k]));
+    h->stage[k] = nullptr;
+    h->stage_bytes[k] = 0;
+    if (hipHostMalloc((void**)&h->stage[k], bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PSDP_ERR_NOMEM, "pinned staging allocation of %zu bytes failed", bytes);
+    }
+    h->stage_bytes[k] = bytes;
+  }
+  if (!h->stage_ev[k]) HIPCHK(hipEventCreateWithFlags(&h->stage_ev[k], hipEventDisableTiming));
+  *out = h->stage[k];
+  return PSDP_OK;
+}
+
+// marks the current slot's copy (already enqueued on the handle's stream)
+static psdp_status_t staging_release(psdp_handle_t h) {
+  const int k = h->stage_next;
+  HIPCHK(hipEventRecord(h->stage_ev[k], h->stream));
+  h->stage_next = (k + 1) % psdp_handle_st::NSTAGE;
+  return PSDP_OK;
+}
 
 // Upper-triangle tiles of a T x T grid, grouped into G x G super-tiles so that
 // tiles running together share operand panels, then interleaved so that the
@@ -735,6 +768,11 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
     (void)hipFree(kv.second.first.dev);
     (void)hipFree(kv.second.second.dev);
   }
+  for (int k = 0; k < psdp_handle_st::NSTAGE; ++k) {
+    if (h->stage_ev[k]) (void)hipEventSynchronize(h->stage_ev[k]);
+    if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
+    if (h->stage[k]) (void)hipHostFree(h->stage[k]);
+  }
   if (h->arena) (void)hipFree(h->arena);
   if (h->st) (void)hipFree(h->st);
   eig_destroy_blas(h->blas);
@@ -932,15 +970,23 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
 // K length so that the static round-robin over workgroups stays balanced).
 // A block's result is bitwise the same alone and in any batch: the same tile
 // code, k order and bound kernels run on it either way.
+// Blocks with n >= BIG_BLOCK_N use 128 x 128 tiles (their own grouped launch
+// per stage), smaller ones 64 x 64: a block's tile size depends on its own n
+// only, so its result does not depend on the batch.  Measured on 256 blocks
+// of n in [32, 1024]: moving the n >= 512 blocks to 128 x 128 tiles made the
+// batch 9% slower (short K loops, two launches in series), hence 2048.
+static constexpr int BIG_BLOCK_N = 2048;
+
 struct BatchPlan {
   Schedule sch;
   int iters = 20;
   double tol = 1e-10, margin = 1.0;
-  int sub = 2, bt = 64, count = 0, nstages = 0, ntiles = 0, max_npad = 0;
+  int count = 0, nstages = 0, max_npad = 0;
+  int ntiles_big = 0, ntiles_small = 0;  // 128 x 128 tiles first, then 64 x 64
   int64_t max_N = 0;
   size_t off_states = 0, off_desc = 0, total = 0;
-  size_t d_lz = 0, d_pad = 0, d_split = 0, d_prod = 0, d_tiles = 0;
-  std::vector<char> blob;  // descriptors, built for one workspace base
+  size_t d_states = 0, d_lz = 0, d_pad = 0, d_split = 0, d_prod = 0, d_tiles = 0;
+  std::vector<char> blob;  // initial states + descriptors, built for one workspace base
 };
 
 // Lays out the workspace and, when base is given, builds every descriptor for
@@ -960,7 +1006,6 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     if (!blocks[b].X || !blocks[b].Y || blocks[b].n <= 0 || blocks[b].n > 65536)
       return fail(PSDP_ERR_INVALID_ARG, "block %d: bad pointer or size", b);
   bp.count = count;
-  const int bt = bp.bt;
   bool any_f32 = false;
   int pa = pieces_of(sch.final_fmt), pp = 0;
   for (int i = 0; i < sch.nsteps; ++i) {
@@ -982,6 +1027,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     q.p2 = q.x32 = kNone;
     for (int i = 0; i < 3; ++i) q.ap[i] = q.p2p[i] = q.p3p[i] = q.xp[i] = kNone;
     q.n = (int)blocks[b].n;
+    const int bt = q.n >= BIG_BLOCK_N ? 128 : 64;
     q.npad = (q.n + bt - 1) / bt * bt;
     q.iters = std::min(bp.iters, q.n);
     bp.max_npad = std::max(bp.max_npad, q.npad);
@@ -999,20 +1045,27 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     q.vec = off;
     off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
   }
-  std::vector<TileRef> tiles;
+  // tiles of each class, the longest K first so that the workgroups' static
+  // round-robin over the list stays balanced
+  std::vector<TileRef> tiles, small;
   for (int b = 0; b < count; ++b) {
-    const int T = bb[b].npad / bt;
+    const bool big = bb[b].n >= BIG_BLOCK_N;
+    const int T = bb[b].npad / (big ? 128 : 64);
     for (int i = 0; i < T; ++i)
-      for (int j = i; j < T; ++j) tiles.push_back(TileRef{b, i, j, 0});
+      for (int j = i; j < T; ++j) (big ? tiles : small).push_back(TileRef{b, i, j, 0});
   }
-  std::stable_sort(tiles.begin(), tiles.end(),
-                   [&](const TileRef& x, const TileRef& y) { return bb[x.mat].npad > bb[y.mat].npad; });
-  bp.ntiles = (int)tiles.size();
+  auto by_k = [&](const TileRef& x, const TileRef& y) { return bb[x.mat].npad > bb[y.mat].npad; };
+  std::stable_sort(tiles.begin(), tiles.end(), by_k);
+  std::stable_sort(small.begin(), small.end(), by_k);
+  bp.ntiles_big = (int)tiles.size();
+  bp.ntiles_small = (int)small.size();
+  tiles.insert(tiles.end(), small.begin(), small.end());
   bp.nstages = 3 * sch.nsteps + 1;
   bp.off_states = off;
   off += align256(sizeof(LanczosState) * count);
   bp.off_desc = off;  // everything below is uploaded in one copy
   size_t doff = 0;
+  bp.d_states = doff, doff += align256(sizeof(LanczosState) * count);  // initial states (copied to off_states)
   bp.d_lz = doff, doff += align256(sizeof(LanczosJob) * count);
   bp.d_pad = doff, doff += align256(sizeof(PadJob) * count);
   bp.d_split = doff, doff += align256(sizeof(SplitJob) * count);
@@ -1024,6 +1077,8 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   LanczosState* states = (LanczosState*)(base + bp.off_states);
   bp.blob.assign(doff, 0);
   char* blob = bp.blob.data();
+  LanczosState* st0 = (LanczosState*)(blob + bp.d_states);  // scale 1 (used as is without auto_scale)
+  for (int b = 0; b < count; ++b) st0[b].scale = st0[b].inv_scale = st0[b].up = 1.0;
   LanczosJob* lz = (LanczosJob*)(blob + bp.d_lz);
   PadJob* pj = (PadJob*)(blob + bp.d_pad);
   SplitJob* sj = (SplitJob*)(blob + bp.d_split);
@@ -1171,18 +1226,18 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   LanczosState* states = (LanczosState*)(base + bp.off_states);
   char* ddev = base + bp.off_desc;
 
-  // initial states: scale 1 (used as is when auto_scale is off)
-  std::vector<LanczosState> st0(count);
-  memset(st0.data(), 0, sizeof(LanczosState) * count);
-  for (auto& x : st0) x.scale = x.inv_scale = 1.0, x.up = 1.0;
-  HIPCHK(hipMemcpyAsync(states, st0.data(), sizeof(LanczosState) * count, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(ddev, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope below
+  // descriptors and initial states go up from a pinned staging slot; the
+  // host waits only if that slot's previous copy has not run yet
+  char* stage = nullptr;
+  STCHK(staging_slot(h, bp.blob.size(), &stage));
+  memcpy(stage, bp.blob.data(), bp.blob.size());
+  HIPCHK(hipMemcpyAsync(ddev, stage, bp.blob.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(states, ddev + bp.d_states, sizeof(LanczosState) * count, hipMemcpyDeviceToDevice, s));
+  STCHK(staging_release(h));
 
   const LanczosJob* lz_d = (const LanczosJob*)(ddev + bp.d_lz);
   const ProdDesc* pd_d = (const ProdDesc*)(ddev + bp.d_prod);
   const TileRef* tl_d = (const TileRef*)(ddev + bp.d_tiles);
-  const int grid = std::min(bp.ntiles, 1024);
   {
     ProfScope ps(h, PSDP_PROF_PREP);
     HIPCHK(launch_sym_pad_batched((const PadJob*)(ddev + bp.d_pad), count, bp.max_npad, s));
@@ -1201,7 +1256,12 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
     const int f = fin ? sch.final_fmt : sch.fmt[st / 3];
     const int avg = fin ? sch.final_avg : (st % 3 != 0 ? sch.avg[st / 3] : 0);
     ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-    HIPCHK(launch_sym_product_grouped(f, bp.sub, avg, pd_d + (size_t)st * count, tl_d, bp.ntiles, grid, s));
+    const ProdDesc* pd = pd_d + (size_t)st * count;
+    if (bp.ntiles_big)
+      HIPCHK(launch_sym_product_grouped(f, 4, avg, pd, tl_d, bp.ntiles_big, std::min(bp.ntiles_big, h->cus), s));
+    if (bp.ntiles_small)
+      HIPCHK(launch_sym_product_grouped(f, 2, avg, pd, tl_d + bp.ntiles_big, bp.ntiles_small,
+                                        std::min(bp.ntiles_small, 4 * h->cus), s));
   }
   if (cfg->synchronize) {
     std::vector<LanczosState> out(count);

@@ -822,6 +822,25 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
 
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,
                                       int ntiles, int grid, hipStream_t s) {
+  if (sub == 4 && !avg) {
+    switch (fmt) {
+      case FMT_F16: return launch_dma_grouped<FMT_F16, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_F16X3: return launch_dma_grouped<FMT_F16X3, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_BF16: return launch_dma_grouped<FMT_BF16, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 4>(descs, tiles, ntiles, grid, s);
+      default: break;  // FP32 and averaged products: the 64 x 64 kernels below
+    }
+  }
+  if (sub == 4) {
+    if (avg) {
+      if (fmt == FMT_F16) return launch_grouped<FMT_F16, 4, true>(descs, tiles, ntiles, grid, s);
+      if (fmt == FMT_BF16) return launch_grouped<FMT_BF16, 4, true>(descs, tiles, ntiles, grid, s);
+      return hipErrorInvalidValue;
+    }
+    if (fmt == FMT_F32) return launch_grouped<FMT_F32, 4, false>(descs, tiles, ntiles, grid, s);
+    return hipErrorInvalidValue;
+  }
   if (sub != 2) return hipErrorInvalidValue;
   if (avg) {
     if (fmt == FMT_F16) return launch_grouped<FMT_F16, 2, true>(descs, tiles, ntiles, grid, s);
