@@ -66,6 +66,46 @@ def measured_traffic(n, fmt):
         return None
 
 
+def precision_study(args, P, W, torch, h, X, Y, Pm, out):
+    """Config 3: the FP32-table chain with FP32 MFMA, BF16x3 and BF16x9 split
+    products, and e4m3 operands (per-tensor scale 2^8) on the first half of the
+    steps with bfloat16 on the rest; next to the shipped schedule.  e4m3
+    operand values are computed on the binary16 MFMA (products of e4m3 values
+    are exact there), so its time is that of binary16, not of FP8 MFMA."""
+    variants = [
+        ("fp32 table, BF16x3 split products", dict(method="custom", table="fp32",
+                                                   step_format=["bf16x3"] * 10, final_format="bf16x3")),
+        ("fp32 table, BF16x9 split products (composite_FP32_emulated)", dict(method="composite_FP32_emulated")),
+        ("fp32 table, e4m3 steps 0-4 + bf16 steps 5-9 and final", dict(method="custom", table="fp32",
+                                                                      step_format=["e4m3"] * 5 + ["bf16"] * 5,
+                                                                      final_format="bf16")),
+        ("fp16 table, e4m3 steps 0-2 + bf16 steps 3-6 and final", dict(method="custom", table="fp16",
+                                                                      step_format=["e4m3"] * 3 + ["bf16"] * 4,
+                                                                      final_format="bf16")),
+    ]
+    res = []
+    if "fp32_chain" in out:
+        res.append({"variant": "fp32 table, FP32 MFMA (composite_FP32)", "ms": out["fp32_chain"]["ms"],
+                    "rel_err": out["fp32_chain"]["rel_err"]})
+    for name, kw in variants:
+        m = kw.pop("method")
+        cfg = P.default_config(m, synchronize=0, **kw)
+        P.project(h, X, out=Y, config=cfg)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        P.project(h, X, out=Y, config=cfg)
+        b.record()
+        torch.cuda.synchronize()
+        err = W.rel_fro_error(Y, Pm)
+        res.append({"variant": name, "ms": round(a.elapsed_time(b), 3), "rel_err": err,
+                    "meets_1e-3": bool(err <= 1e-3)})
+        log(f"precision study: {name}: {res[-1]['ms']} ms, rel err {err:.3e}")
+    res.append({"variant": "shipped mixed (fp16 table, FP16x3 steps 0-3, binary16 rest)", "ms": out["value"],
+                "rel_err": out["rel_err"], "meets_1e-3": bool(out["rel_err"] <= 1e-3)})
+    return res
+
+
 MIXED_UNITS = 12 * 3 + 10  # mixed schedule: 12 FP16x3 products (3 MFMA products each) + 10 binary16
 
 
@@ -173,6 +213,8 @@ def main():
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=6144, help="CPU baseline block size (0: skip)")
+    ap.add_argument("--precision-study", type=int, default=1,
+                    help="config 3 variants (BF16x3, BF16x9, e4m3 first half + bf16) at the same size (0: skip)")
     ap.add_argument("--batched", type=int, default=256, help="config 4 block count (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="config 5 requests in the timed Poisson stream (0: skip; the full config is 10000)")
@@ -291,6 +333,9 @@ def main():
         err32 = W.rel_fro_error(Y, Pm)
         log(f"composite_FP32: {ms32:.2f} ms/projection, rel err {err32:.3e}")
         out["fp32_chain"] = {"ms": round(ms32, 3), "rel_err": err32, "speedup": round(ms32 / ms, 2)}
+
+    if args.precision_study:
+        out["precision_study"] = precision_study(args, P, W, torch, h, X, Y, Pm, out)
 
     del X, Y, Pm
     torch.cuda.empty_cache()

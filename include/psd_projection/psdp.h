@@ -56,7 +56,9 @@ typedef enum psdp_format {
   PSDP_FMT_F16X3 = 2,  /* 2^8 v = hi + lo binary16 split, 3 products (~22-bit)     */
   PSDP_FMT_BF16 = 3,   /* bfloat16 operands                                        */
   PSDP_FMT_BF16X3 = 4, /* hi + lo bfloat16 split, 3 products (~16-bit)             */
-  PSDP_FMT_BF16X9 = 5  /* hi + mid + lo bfloat16 split, 9 products (~fp32)         */
+  PSDP_FMT_BF16X9 = 5, /* hi + mid + lo bfloat16 split, 9 products (~fp32)         */
+  PSDP_FMT_E4M3 = 6    /* FP8 e4m3 operand values, per-tensor scale 2^8 (|a| <= 1  */
+                       /* after the bound), products on the binary16 MFMA          */
 } psdp_format_t;
 
 /* Coefficient table of the composite polynomial chain. */
@@ -142,9 +144,9 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
 /* ---- profiling: HIP events around every launch of the handle, by kind ---- */
 typedef enum psdp_prof_kind {
   PSDP_PROF_PROD_F32 = 0, /* + psdp_format_t: symmetric products by operand format */
-  PSDP_PROF_LANCZOS = 6,  /* the whole spectral bound                                */
-  PSDP_PROF_PREP = 7,     /* symmetrize/pad + scale/split                            */
-  PSDP_PROF_KINDS = 8
+  PSDP_PROF_LANCZOS = 8,  /* the whole spectral bound                                */
+  PSDP_PROF_PREP = 9,     /* symmetrize/pad + scale/split                            */
+  PSDP_PROF_KINDS = 10
 } psdp_prof_kind_t;
 /* on != 0 starts recording (and clears old records); 0 stops. */
 psdp_status_t psdp_profile_enable(psdp_handle_t h, int on);

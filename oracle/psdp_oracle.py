@@ -129,6 +129,27 @@ def round_bf16(x):
 F16X3_SCALE = 256.0  # 2**8: s*x split into hi + lo binary16 keeps lo out of the subnormal range
 
 
+def round_e4m3(x):
+    """float32 -> nearest FP8 E4M3FN value (OCP: 3 mantissa bits, exponent
+    bias 7, normal exponents >= -6, subnormal quantum 2**-9, max 448, values
+    beyond saturate), ties to even -> float32.  Not a reference format (the
+    reference has no FP8 path); the config-3 precision study asks for it."""
+    x = np.asarray(x, dtype=np.float32)
+    a = np.abs(x).astype(np.float64)
+    out = np.empty_like(x)
+    big = ~(a < 448.0)
+    with np.errstate(divide="ignore"):
+        e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    # np.log2 can land one ulp low at exact powers of two: fix the exponent
+    e = np.where(2.0 ** (e + 1) <= a, e + 1, e)
+    q = 2.0 ** (np.maximum(e, -6.0) - 3.0)
+    r = np.rint(x.astype(np.float64) / q) * q  # np.rint: ties to even
+    out[...] = r.astype(np.float32)
+    out[big] = np.copysign(np.float32(448.0), x[big])
+    out[np.isnan(x)] = np.nan
+    return out
+
+
 def split_operand(x, fmt):
     """Return the list of float32 pieces a GEMM operand is stored as."""
     x = np.asarray(x, dtype=np.float32)
@@ -144,6 +165,9 @@ def split_operand(x, fmt):
         return [hi, round_f16(w - hi)]
     if fmt == "bf16":
         return [round_bf16(x)]
+    if fmt == "e4m3":
+        # per-tensor scale 2^8 (operands are bounded by 1 after the spectral bound)
+        return [round_e4m3(x * np.float32(F16X3_SCALE))]
     if fmt == "bf16x3":
         hi = round_bf16(x)
         return [hi, round_bf16(x - hi)]
@@ -164,6 +188,8 @@ def emulated_product(x, y, fmt):
     mm = lambda a, b: (a @ b).astype(np.float32)
     if fmt in ("f32", "f16", "bf16"):
         return mm(px[0], py[0])
+    if fmt == "e4m3":
+        return mm(px[0], py[0]) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
     if fmt == "f16x3":
         acc = mm(px[1], py[0]) + mm(px[0], py[1])
         return (acc + mm(px[0], py[0])) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
@@ -514,4 +540,6 @@ PINNING = {
                         "recomputation from T; lo <= ||A|| on random inputs; zero matrix -> (0,1)",
     "eig_FP64_psd": "diag KAT, PSD fixed point, closed form Q max(L,0) Q^T",
     "GlibcRandom": "first outputs after srand(1) (1804289383, 846930886, ...) and libc rand()",
+    "round_e4m3": "OCP E4M3FN grid KATs (448, 2^-6, 2^-9, ties), brute force over all 256 codes, "
+                  "torch float8_e4m3fn casts in range",
 }

@@ -32,9 +32,9 @@ LIB_PATH = _PKG / "libpsdp.so"
 # psdp_status_t
 OK, ERR_INVALID_ARG, ERR_NOMEM, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_DEVICE = range(6)
 # psdp_format_t
-FMT_F32, FMT_F16, FMT_F16X3, FMT_BF16, FMT_BF16X3, FMT_BF16X9 = range(6)
+FMT_F32, FMT_F16, FMT_F16X3, FMT_BF16, FMT_BF16X3, FMT_BF16X9, FMT_E4M3 = range(7)
 FORMATS = {"f32": FMT_F32, "f16": FMT_F16, "f16x3": FMT_F16X3, "bf16": FMT_BF16,
-           "bf16x3": FMT_BF16X3, "bf16x9": FMT_BF16X9}
+           "bf16x3": FMT_BF16X3, "bf16x9": FMT_BF16X9, "e4m3": FMT_E4M3}
 # psdp_table_t
 TABLE_FP32, TABLE_FP16 = 0, 1
 # psdp_method_t
@@ -45,7 +45,7 @@ METHODS = {"composite_FP32": METHOD_FP32, "composite_FP16": METHOD_FP16,
 MAX_STEPS = 16
 # psdp_prof_kind_t
 PROF_KINDS = {0: "prod_f32", 1: "prod_f16", 2: "prod_f16x3", 3: "prod_bf16", 4: "prod_bf16x3",
-              5: "prod_bf16x9", 6: "lanczos", 7: "prep"}
+              5: "prod_bf16x9", 6: "prod_e4m3", 8: "lanczos", 9: "prep"}
 
 
 class PsdpError(RuntimeError):

@@ -32,6 +32,18 @@ __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) {
   return __uint_as_float(((uint32_t)b) << 16);
 }
 
+// Nearest value of the FP8 e4m3 grid (OCP E4M3FN: 3 mantissa bits, normal
+// exponents down to 2^-6, subnormal quantum 2^-9, largest finite 448; values
+// beyond saturate), ties to even.  Exact in binary16.
+__device__ __forceinline__ float round_e4m3(float w) {
+  const float a = fabsf(w);
+  if (!(a < 448.0f)) return a == a ? copysignf(448.0f, w) : w;  // saturate (NaN stays NaN)
+  int e;
+  frexpf(a, &e);                      // a = m 2^e, m in [0.5, 1)
+  const int q = (e - 1 < -6 ? -6 : e - 1) - 3;
+  return ldexpf(rintf(ldexpf(w, -q)), q);
+}
+
 // Splits a float32 value into the pieces of operand format `fmt`.
 // F16X3: w = 2^8 v (exact), hi = f16(w), lo = f16(w - hi).  BF16X3: hi, v - hi.
 // BF16X9: hi, mid, lo (three bfloat16 terms).
@@ -52,6 +64,7 @@ __device__ __forceinline__ void split_value(int fmt, float v, uint16_t out[3]) {
       out[1] = f32_to_bf16_bits(v - bf16_bits_to_f32(h));
       break;
     }
+    case FMT_E4M3: out[0] = f32_to_f16_bits(round_e4m3(v * F16X3_SCALE)); break;
     case FMT_BF16X9: {
       uint16_t h = f32_to_bf16_bits(v);
       float r = v - bf16_bits_to_f32(h);

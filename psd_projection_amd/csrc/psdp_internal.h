@@ -12,7 +12,8 @@ enum : int {
   FMT_BF16 = 3,    // bfloat16 operands
   FMT_BF16X3 = 4,  // hi + lo bfloat16 split, 3 products
   FMT_BF16X9 = 5,  // hi + mid + lo bfloat16 split, 9 products
-  FMT_COUNT = 6
+  FMT_E4M3 = 6,    // 2^8 v rounded to the FP8 e4m3 grid, held in binary16 (binary16 MFMA)
+  FMT_COUNT = 7
 };
 
 // FMT_F16X3 splits s*v (s = 2^8, exact) into hi + lo binary16 pieces so that

@@ -54,6 +54,11 @@ struct FmtTraits<FMT_BF16X3> {
   static constexpr int P = 2, BK = 32;
 };
 template <>
+struct FmtTraits<FMT_E4M3> {
+  using E = uint16_t;
+  static constexpr int P = 1, BK = 64;
+};
+template <>
 struct FmtTraits<FMT_BF16X9> {
   using E = uint16_t;
   static constexpr int P = 3, BK = 32;
@@ -397,7 +402,7 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
   // ---------------------------------------------------------------- epilogue
   // Accumulators go to LDS with compile-time indices only (they stay in
   // registers), HR rows at a time, then epilogue_rows writes them out.
-  constexpr float PSCALE = FMT == FMT_F16X3 ? F16X3_PROD_SCALE : 1.0f;
+  constexpr float PSCALE = (FMT == FMT_F16X3 || FMT == FMT_E4M3) ? F16X3_PROD_SCALE : 1.0f;
   float* Ct = (float*)smem;
   constexpr int CS = TL::BT + 1;
   constexpr int PHASES = TL::BT / TL::HR;
@@ -670,7 +675,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   lds_barrier();  // every wave is done reading the stages before they hold C
 
   // ---------------------------------------------------------------- epilogue
-  constexpr float PSCALE = FMT == FMT_F16X3 ? F16X3_PROD_SCALE : 1.0f;
+  constexpr float PSCALE = (FMT == FMT_F16X3 || FMT == FMT_E4M3) ? F16X3_PROD_SCALE : 1.0f;
   float* Ct = (float*)smem;
   constexpr int CS = DT::BT + 1;
   constexpr int PHASES = DT::BT / DT::HR;
@@ -754,7 +759,7 @@ static hipError_t launch_grouped(const ProdDesc* descs, const TileRef* tiles, in
 // 256 x 256 tiles (SUB = 8) exist for the 1- and 2-piece 16-bit formats with
 // plain (non-averaged) products.
 bool sym_big_tile_capable(int fmt, int avg) {
-  return !avg && (fmt == FMT_F16 || fmt == FMT_BF16 || fmt == FMT_F16X3 || fmt == FMT_BF16X3);
+  return !avg && (fmt == FMT_F16 || fmt == FMT_BF16 || fmt == FMT_F16X3 || fmt == FMT_BF16X3 || fmt == FMT_E4M3);
 }
 
 // Producer waves (WS) are kept available but off: measured at n = 16384 they
@@ -809,6 +814,7 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
                       : (sub == 2 ? launch_one<FMT_F32, 2, false>(d, tiles, ntiles, s) : hipErrorInvalidValue);
     PSDP_CASE1(FMT_F16)
     PSDP_CASE1(FMT_BF16)
+    PSDP_CASE1(FMT_E4M3)
     PSDP_CASE2(FMT_F16X3)
     PSDP_CASE2(FMT_BF16X3)
     PSDP_CASEN(FMT_BF16X9)
@@ -829,6 +835,7 @@ hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc*
       case FMT_BF16: return launch_dma_grouped<FMT_BF16, 4>(descs, tiles, ntiles, grid, s);
       case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 4>(descs, tiles, ntiles, grid, s);
       case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_E4M3: return launch_dma_grouped<FMT_E4M3, 4>(descs, tiles, ntiles, grid, s);
       default: break;  // FP32 and averaged products: the 64 x 64 kernels below
     }
   }
@@ -854,6 +861,7 @@ hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc*
     case FMT_BF16: return launch_dma_grouped<FMT_BF16, 2>(descs, tiles, ntiles, grid, s);
     case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 2>(descs, tiles, ntiles, grid, s);
     case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_E4M3: return launch_dma_grouped<FMT_E4M3, 2>(descs, tiles, ntiles, grid, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -54,7 +54,7 @@ def start_vector(seed, n):
 
 # ------------------------------------------------------------------ products
 @pytest.mark.parametrize("fmt,avg", [("f32", 0), ("f16", 0), ("f16x3", 0), ("bf16", 0), ("bf16x3", 0),
-                                     ("bf16x9", 0), ("f16", 1), ("bf16", 1)])
+                                     ("bf16x9", 0), ("e4m3", 0), ("f16", 1), ("bf16", 1)])
 @pytest.mark.parametrize("n", [64, 200, 1100, 2100])
 def test_sym_product(handle, fmt, avg, n):
     # The kernel computes tiles with bi <= bj only; its result is the upper

@@ -305,3 +305,38 @@ def test_lanczos_invariant_start_vector():
     d = np.array([-4.0, 1.0, 2.0, 3.0])
     lo, up = O.lanczos_two_norm(np.diag(d), 20, 1e-10, q0=np.eye(4)[0])
     assert lo == pytest.approx(4.0) and up == pytest.approx(np.sqrt(16 + 20))
+
+
+# ------------------------------------------------------------------ e4m3 grid
+def test_round_e4m3_known_answers():
+    v = np.array([0.0, 448.0, 460.0, 1e6, -1e6, 2.0 ** -6, 2.0 ** -9, 2.0 ** -10, 1.5 * 2.0 ** -10,
+                  1.0 + 1.0 / 16, 1.0 + 3.0 / 16, 17.0, 19.0], dtype=np.float32)
+    want = np.array([0.0, 448.0, 448.0, 448.0, -448.0, 2.0 ** -6, 2.0 ** -9, 0.0, 2.0 ** -9,
+                     1.0, 1.25, 16.0, 20.0], dtype=np.float32)
+    # 2^-10 is a tie between 0 and 2^-9 (even: 0); 1 + 1/16 ties to 1, 1 + 3/16 to 1.25;
+    # 17 = 16 + 1 ties to 16 (quantum 2 above 16), 19 to 20
+    assert np.array_equal(O.round_e4m3(v), want)
+
+
+def test_round_e4m3_is_the_code_grid():
+    torch = pytest.importorskip("torch")
+    codes = torch.arange(256, dtype=torch.uint8).view(torch.float8_e4m3fn).float().numpy()
+    fin = codes[np.isfinite(codes)]
+    assert len(fin) == 254 and fin.max() == 448.0
+    assert np.array_equal(O.round_e4m3(fin), fin)
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(50000) * rng.choice([1e-3, 1.0, 30.0, 300.0], 50000)).astype(np.float32)
+    x = x[np.abs(x) < 440]
+    assert np.array_equal(O.round_e4m3(x), torch.from_numpy(x).to(torch.float8_e4m3fn).float().numpy())
+    # planted mistake: truncation instead of round-to-nearest fails
+    trunc = np.sign(x) * np.floor(np.abs(x) * 8) / 8
+    assert not np.array_equal(O.round_e4m3(x), trunc)
+
+
+def test_e4m3_product_error_band():
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (64, 64)).astype(np.float32) / 8
+    exact = X.astype(np.float64) @ X.astype(np.float64)
+    e = np.linalg.norm(O.emulated_product(X, X, "e4m3") - exact) / np.linalg.norm(exact)
+    # 3 mantissa bits: relative operand error <= 2^-4, product error a few 1e-2
+    assert 5e-3 < e < 6e-2, e
