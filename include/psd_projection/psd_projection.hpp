@@ -106,6 +106,58 @@ inline void eig_FP32_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat
   detail::report(psdp_eig_fp32_psd(h, mat, (int64_t)mat_size, eigenvalues_out), "eig_FP32_psd");
 }
 
+// composite_FP32.h:36-45.  verbose: diagnostics come back through the
+// status and psdp_last_error() instead of stdout.
+inline void composite_FP32_auto_scale_deflate(psdp_handle_t h, psdp_handle_t /*solverH*/, double* mat,
+                                              const int mat_size, const size_t k = 40, const double tol = 1e-10,
+                                              const double ortho_tol = 1e-3, const bool /*verbose*/ = false) {
+  detail::report(psdp_composite_fp32_auto_scale_deflate(h, mat, mat_size, (int64_t)k, tol, ortho_tol),
+                 "composite_FP32_auto_scale_deflate");
+}
+
+// composite_FP16.h:39-50
+inline void composite_FP16_auto_scale_deflate(psdp_handle_t h, psdp_handle_t /*solverH*/, double* mat,
+                                              const int mat_size, float* float_workspace = nullptr,
+                                              void* /*half_workspace*/ = nullptr, const size_t k = 40,
+                                              const int maxiter = 100, const double tol = 1e-10,
+                                              const bool /*verbose*/ = false) {
+  const size_t b = detail::usable_ws(PSDP_METHOD_COMPOSITE_FP16, mat_size, float_workspace,
+                                     3 * sizeof(float) * (size_t)mat_size * mat_size);
+  detail::report(psdp_composite_fp16_auto_scale_deflate(h, mat, mat_size, b ? float_workspace : nullptr, b,
+                                                        (int64_t)k, maxiter, tol),
+                 "composite_FP16_auto_scale_deflate");
+}
+
+// eig_FP64_psd.h:22-31
+inline void eig_FP64_deflate(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
+                             const size_t k = 40, const int maxiter = 100, const double tol = 1e-10,
+                             const bool /*verbose*/ = false) {
+  detail::report(psdp_eig_fp64_deflate(h, mat, (int64_t)mat_size, (int64_t)k, maxiter, tol), "eig_FP64_deflate");
+}
+
+// lanczos.h:33-43.  Returns the bound, as the reference does; *r receives the
+// number of pairs kept (eigenvalues: k device values, eigenvectors: n x k).
+inline double compute_eigenpairs(psdp_handle_t h, psdp_handle_t /*cusolverH*/, const double* A, size_t n,
+                                 const size_t k, size_t* r, double* eigenvalues, double* eigenvectors,
+                                 const bool upper_bound_only = false, const size_t max_iter = 0,
+                                 const double tol = 1e-10, const double ortho_tol = 1e-5,
+                                 const bool /*verbose*/ = false) {
+  int64_t rr = 0;
+  double bound = 0.0;
+  detail::report(psdp_compute_eigenpairs(h, A, (int64_t)n, (int64_t)k, &rr, eigenvalues, eigenvectors,
+                                         upper_bound_only ? 1 : 0, (int64_t)max_iter, tol, ortho_tol, &bound),
+                 "compute_eigenpairs");
+  if (r && !upper_bound_only) *r = (size_t)rr;
+  return bound;
+}
+
+// lobpcg.h:16-28 (D receives m values)
+inline void lobpcg(psdp_handle_t h, psdp_handle_t /*cusolverH*/, const double* A, double* V, double* D, const int n,
+                   const int m, const bool warmstart = false, const int maxiter = 100, const double tol = 1e-8,
+                   const bool /*verbose*/ = false) {
+  detail::report(psdp_lobpcg(h, A, V, D, n, m, warmstart ? 1 : 0, maxiter, tol), "lobpcg");
+}
+
 }  // namespace psd_projection
 
 #endif  // PSD_PROJECTION_PSD_PROJECTION_HPP

@@ -21,6 +21,15 @@
  *   psdp_approximate_two_norm            <- approximate_two_norm        lanczos.h:11-17
  *   psdp_eig_fp64_psd                    <- eig_FP64_psd                eig_FP64_psd.h:14-20
  *   psdp_eig_fp32_psd                    <- eig_FP32_psd                eig_FP32_psd.h:14-20
+ *   psdp_compute_eigenpairs              <- compute_eigenpairs          lanczos.h:33-43
+ *   psdp_lobpcg                          <- lobpcg                      lobpcg.h:16-28
+ *   psdp_composite_fp32_auto_scale_deflate
+ *                                        <- composite_FP32_auto_scale_deflate
+ *                                                                       composite_FP32.h:36-45
+ *   psdp_composite_fp16_auto_scale_deflate
+ *                                        <- composite_FP16_auto_scale_deflate
+ *                                                                       composite_FP16.h:39-50
+ *   psdp_eig_fp64_deflate                <- eig_FP64_deflate            eig_FP64_psd.h:22-31
  *
  * Additions with no reference counterpart: float32 in/out projections with a
  * per-step precision schedule (psdp_project_f32/_f64), grouped projection of
@@ -146,7 +155,10 @@ typedef enum psdp_prof_kind {
   PSDP_PROF_PROD_F32 = 0, /* + psdp_format_t: symmetric products by operand format */
   PSDP_PROF_LANCZOS = 8,  /* the whole spectral bound                                */
   PSDP_PROF_PREP = 9,     /* symmetrize/pad + scale/split                            */
-  PSDP_PROF_KINDS = 10
+  PSDP_PROF_EIGENPAIRS = 10, /* psdp_compute_eigenpairs (whole call)                */
+  PSDP_PROF_LOBPCG = 11,  /* psdp_lobpcg (whole call)                                */
+  PSDP_PROF_RANK_UPDATE = 12, /* X += V diag(c) V^T of the deflated projections      */
+  PSDP_PROF_KINDS = 13
 } psdp_prof_kind_t;
 /* on != 0 starts recording (and clears old records); 0 stops. */
 psdp_status_t psdp_profile_enable(psdp_handle_t h, int on);
@@ -171,6 +183,58 @@ psdp_status_t psdp_workspace_size_batched(const psdp_block_t* blocks, int count,
  * receives the eigenvalues of the input (ascending) before clamping. */
 psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
 psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
+
+/* ---- extremal eigenpairs and deflated projections (float64, device arrays) ---- */
+
+/* Residual of Ritz pair j in psdp_compute_eigenpairs: |beta_last Z(row, j)|
+ * for the eigenvector matrix Z of the Lanczos tridiagonal T. */
+typedef enum psdp_ritz_residual {
+  PSDP_RITZ_RESIDUAL_FIRST_ROW = 0, /* row 0, as src/lanczos.cu:295-301 reads it (default)       */
+  PSDP_RITZ_RESIDUAL_LAST_ROW = 1   /* row nb-1: the Lanczos residual ||A x - theta x||          */
+} psdp_ritz_residual_t;
+/* Handle options of psdp_compute_eigenpairs: the residual row, and whether
+ * the kept Ritz vectors Q z are divided by their norms (default 0: the
+ * reference's unnormalised vectors).  The deflated FP32 projection uses the
+ * residual row set here and always normalises. */
+psdp_status_t psdp_set_eigenpair_options(psdp_handle_t h, int residual_row, int normalize_vectors);
+
+/* compute_eigenpairs: Lanczos with the whole basis kept (max_iter = 0 -> n
+ * iterations), Ritz pairs in order of decreasing |theta| kept while their
+ * residual is < ortho_tol and their overlap with every kept vector is
+ * <= ortho_tol, at most k.  eigenvalues (k) and eigenvectors (n x k, the
+ * unnormalised Ritz vectors Q z) are device arrays; *r (host) receives the
+ * number kept.  *upper_bound (host) = |theta_max| + residual + 1e-3, the
+ * reference's return value.  upper_bound_only != 0 computes only the bound
+ * (r, eigenvalues, eigenvectors may then be NULL).  tol is accepted and
+ * unused, as in the reference.  Blocks until done. */
+psdp_status_t psdp_compute_eigenpairs(psdp_handle_t h, const double* A, int64_t n, int64_t k, int64_t* r,
+                                      double* eigenvalues, double* eigenvectors, int upper_bound_only,
+                                      int64_t max_iter, double tol, double ortho_tol, double* upper_bound);
+
+/* lobpcg: the m largest eigenpairs of A (3 m <= n).  V: n x m device array
+ * (output; with warmstart != 0 also the orthonormal start block), D: m
+ * device values in descending order (the reference documents D as m x m but
+ * writes m values).  Stops when ||A X - X diag(D)||_F < tol or after maxiter
+ * iterations (reference defaults: 100, 1e-8).  Blocks until done. */
+psdp_status_t psdp_lobpcg(psdp_handle_t h, const double* A, double* V, double* D, int64_t n, int64_t m,
+                          int warmstart, int maxiter, double tol);
+/* Iterations and final residual norm of the handle's last lobpcg run. */
+psdp_status_t psdp_lobpcg_last(psdp_handle_t h, int* iterations, double* residual_norm);
+
+/* Deflated projections, in place (reference defaults: k = 40, tol = 1e-10,
+ * ortho_tol = 1e-3, maxiter = 100).  Blocks until done.
+ * fp32: the k largest-|lambda| Lanczos pairs (unit vectors) are removed, the rest is scaled
+ * by s = 1.1 up + 1e-5 (up: Lanczos bound of the remainder) and projected by
+ * the FP32 chain, the positive pairs are added back.
+ * fp16 / eig_fp64: the k largest and the k smallest LOBPCG pairs (3 k <= n)
+ * are removed, the rest is projected by the FP16 chain (after the same
+ * scaling, from approximate_two_norm) or by the float64 EVD, the positive
+ * pairs are added back.  ws/ws_bytes: optional workspace of the FP16 chain. */
+psdp_status_t psdp_composite_fp32_auto_scale_deflate(psdp_handle_t h, double* mat, int64_t n, int64_t k,
+                                                     double tol, double ortho_tol);
+psdp_status_t psdp_composite_fp16_auto_scale_deflate(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                                     size_t ws_bytes, int64_t k, int maxiter, double tol);
+psdp_status_t psdp_eig_fp64_deflate(psdp_handle_t h, double* mat, int64_t n, int64_t k, int maxiter, double tol);
 
 #ifdef __cplusplus
 }

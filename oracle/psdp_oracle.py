@@ -13,8 +13,8 @@ holds the checks.
 
 Reference files (ComputationalRobotics/psd_projection):
   src/composite_FP32.cu, src/composite_FP16.cu, src/composite_FP32_emulated.cu,
-  src/lanczos.cu, src/eig_FP64_psd.cu, src/eig_FP32_psd.cu, src/utils.cu,
-  test/test_utils.hpp
+  src/lanczos.cu, src/lobpcg.cu, src/eig_FP64_psd.cu, src/eig_FP32_psd.cu,
+  src/utils.cu, test/test_utils.hpp
 """
 from __future__ import annotations
 
@@ -421,6 +421,190 @@ def rel_fro_error(Y, P):
 
 
 # ----------------------------------------------------------------------------
+# Extremal eigenpairs and deflated projections
+# ----------------------------------------------------------------------------
+
+def lanczos_eigenpairs(A, k, q0, max_iter=0, ortho_tol=1e-5, upper_bound_only=False, residual_row="first",
+                       normalize=False):
+    """Restatement of compute_eigenpairs (src/lanczos.cu:319-547).
+
+    Lanczos on A with the whole basis kept, no reorthogonalisation
+    (:359-389): w = A q_m; w -= beta_{m-1} q_{m-1}; alpha_m = q_m . w;
+    w -= alpha_m q_m; beta_m = ||w||; stop if beta_m <= DBL_EPSILON (the
+    iteration counts); q_{m+1} = w * (1/beta_m).  max_iter = 0 means n
+    (:331-332); max_iter may exceed n.  T = tridiag(alpha, beta[:nb-1]),
+    eigenpairs ascending (:391-423).  Residuals |beta_last Z(row, j)|, row 0
+    as the reference reads Z[j*m] (:295-301), or the last row.  Order by
+    decreasing |theta| (:438-447); keep a pair when res < ortho_tol and
+    |x_a . x| <= ortho_tol for every kept x_a, x = Q z unnormalised, until k
+    (:453-507).  bound = |theta_max| + res_max + 1e-3 (:511-547).
+    normalize divides the kept vectors by their norms after the selection
+    (not in the reference; the deflated projection needs unit vectors).
+    Returns (values, vectors (n x r), bound, state).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    max_iter = n if int(max_iter) == 0 else int(max_iter)
+    q = np.array(q0, dtype=np.float64)
+    nq = np.linalg.norm(q)
+    if nq != 0.0:
+        q = q * (1.0 / nq)
+    Q = np.zeros((n, max_iter + 1))
+    Q[:, 0] = q
+    alpha, beta = [], []
+    for m in range(max_iter):
+        w = A @ Q[:, m]
+        if m > 0:
+            w = w - beta[-1] * Q[:, m - 1]
+        a = Q[:, m] @ w
+        alpha.append(a)
+        w = w - a * Q[:, m]
+        b = np.linalg.norm(w)
+        beta.append(b)
+        if b <= np.finfo(np.float64).eps:
+            break
+        Q[:, m + 1] = w * (1.0 / b)
+    nb = len(alpha)
+    T = np.diag(alpha) + np.diag(beta[: nb - 1], 1) + np.diag(beta[: nb - 1], -1)
+    theta, Z = np.linalg.eigh(T)
+    row = 0 if residual_row == "first" else nb - 1
+    res = np.abs(beta[-1] * Z[row, :])
+    idx = sorted(range(nb), key=lambda j: -abs(theta[j]))
+    bound = abs(theta[idx[0]]) + res[idx[0]] + 1e-3
+    state = dict(nb=nb, alpha=np.array(alpha), beta=np.array(beta), theta=theta, Z=Z, res=res, Q=Q[:, :nb])
+    if upper_bound_only:
+        return None, None, bound, state
+    vals, vecs = [], []
+    for j in idx:
+        if len(vals) >= k:
+            break
+        if not res[j] < ortho_tol:
+            continue
+        x = Q[:, :nb] @ Z[:, j]
+        if vecs and np.max(np.abs(np.array(vecs) @ x)) > ortho_tol:
+            continue
+        vals.append(theta[j])
+        vecs.append(x)
+    V = np.array(vecs).T if vecs else np.zeros((n, 0))
+    if normalize and vecs:
+        V = V / np.linalg.norm(V, axis=0)
+    return np.array(vals), V, bound, state
+
+
+def lobpcg(A, m, X0, maxiter=100, tol=1e-8, warm=False, sign=1.0):
+    """Restatement of lobpcg (src/lobpcg.cu:12-256) for the m largest
+    eigenpairs of sign*A.
+
+    Start: X0 orthonormalised by Householder QR unless warm (:84-104), then
+    Rayleigh-Ritz on it (:106-133).  Iteration (:135-218): R = AX - X lam
+    (X lam rounded first, as Ddgmm + Daxpy), stop when ||R||_F < tol;
+    S = [X, R, dX]; Q = Householder QR of S; T = Q^T A Q symmetrised;
+    the m largest Ritz pairs, descending: X' = Q Z_m, dX = X' - X.
+    Stated differences, shared with the library and not changing the limit:
+    the first iteration stacks [X, R] (the reference's [X, R, X] adds a
+    dependent block), the initial Rayleigh-Ritz pairs vectors with their own
+    values, and AX' is carried as (A Q) Z_m.
+    Returns (V (n x m), lam (m, descending), iterations, ||R||_F).
+    """
+    A = sign * np.asarray(A, dtype=np.float64)
+    X = np.array(X0, dtype=np.float64)
+    if not warm:
+        X, _ = np.linalg.qr(X, mode="reduced")
+
+    def rayleigh_ritz(Qb):
+        AQ = A @ Qb
+        T = Qb.T @ AQ
+        T = 0.5 * T + 0.5 * T.T
+        th, Z = np.linalg.eigh(T)
+        Zm = Z[:, ::-1][:, :m]
+        return Qb @ Zm, AQ @ Zm, th[::-1][:m].copy()
+
+    X, AX, lam = rayleigh_ritz(X)
+    dX = None
+    iters, rn = 0, 0.0
+    for it in range(1, int(maxiter) + 1):
+        R = AX - X * lam
+        rn = float(np.linalg.norm(R))
+        if rn < tol:
+            break
+        iters = it
+        S = np.hstack([X, R] if dX is None else [X, R, dX])
+        Qs, _ = np.linalg.qr(S, mode="reduced")
+        Xn, AXn, lam = rayleigh_ritz(Qs)
+        dX = Xn - X
+        X, AX = Xn, AXn
+    return X, lam, iters, rn
+
+
+def _rank_update(X, V, c):
+    """X + V diag(c) V^T (the Dger loops of the deflated projections)."""
+    if V.shape[1] == 0:
+        return X
+    return X + (V * c) @ V.T
+
+
+def composite_FP32_auto_scale_deflate(X, k=40, ortho_tol=1e-3, q0=None, q0_bound=None, residual_row="first",
+                                      chain=None, normalize=True):
+    """src/composite_FP32.cu:193-254: the k largest-|lambda| Lanczos pairs
+    (compute_eigenpairs, max_iter = n) removed; bound of the remainder by
+    compute_eigenpairs(upper_bound_only, max_iter 100, ortho 1e-5); s =
+    1.1 up + 1e-5 if up > 0 else 1; s * composite_FP32(X / s); positive
+    pairs added back.  k = 0 deflates nothing.  normalize=False keeps the
+    reference's unnormalised Ritz vectors (src/lanczos.cu:478-481), whose
+    x x^T removes theta ||x||^2 once the basis has lost orthogonality."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    q0 = np.ones(n) if q0 is None else q0
+    q0_bound = q0 if q0_bound is None else q0_bound
+    if k > 0:
+        vals, V, _, _ = lanczos_eigenpairs(X, k, q0, 0, ortho_tol, residual_row=residual_row, normalize=normalize)
+    else:
+        vals, V = np.zeros(0), np.zeros((n, 0))
+    Xd = _rank_update(X, V, -vals)
+    _, _, up, _ = lanczos_eigenpairs(Xd, 0, q0_bound, 100, 1e-5, upper_bound_only=True, residual_row=residual_row)
+    s = 1.1 * up + 1e-5 if up > 0.0 else 1.0
+    f = composite_FP32 if chain is None else chain
+    Y = f(Xd * (1.0 / s)) * s
+    return _rank_update(Y, V, np.maximum(vals, 0.0))
+
+
+def _lobpcg_deflate(X, k, maxiter, tol, X0):
+    n = X.shape[0]
+    Vmax, lmax, _, _ = lobpcg(X, k, X0, maxiter, tol)
+    X = _rank_update(X, Vmax, -lmax)
+    Vmin, mu, _, _ = lobpcg(X, k, X0, maxiter, tol, sign=-1.0)
+    X = _rank_update(X, Vmin, mu)
+    return X, np.hstack([Vmax, Vmin]), np.concatenate([lmax, -mu])
+
+
+def composite_FP16_auto_scale_deflate(X, k=40, maxiter=100, tol=1e-10, X0=None, q0=None, chain=None):
+    """src/composite_FP16.cu:276-380: LOBPCG removes the k largest pairs,
+    then the k smallest of the remainder (largest of its negation, :316-333);
+    approximate_two_norm bound, s = 1.1 up + 1e-5 if up > 0 else 1;
+    s * composite_FP16(X / s); positive pairs of both sets added back."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    X0 = np.random.default_rng(0).random((n, k)) if X0 is None else X0
+    Xd, V, lam = _lobpcg_deflate(X, k, maxiter, tol, X0)
+    _, up = lanczos_two_norm(Xd, 20, 1e-10, q0)
+    s = 1.1 * up + 1e-5 if up > 0.0 else 1.0
+    f = composite_FP16 if chain is None else chain
+    Y = f(Xd * (1.0 / s)) * s
+    return _rank_update(Y, V, np.maximum(lam, 0.0))
+
+
+def eig_FP64_deflate(X, k=40, maxiter=100, tol=1e-10, X0=None):
+    """src/eig_FP64_psd.cu:87-177: the LOBPCG deflation of
+    composite_FP16_auto_scale_deflate, eig_FP64_psd of the remainder,
+    positive pairs added back."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    X0 = np.random.default_rng(0).random((n, k)) if X0 is None else X0
+    Xd, V, lam = _lobpcg_deflate(X, k, maxiter, tol, X0)
+    return _rank_update(eig_FP64_psd(Xd), V, np.maximum(lam, 0.0))
+
+
+# ----------------------------------------------------------------------------
 # Input generators
 # ----------------------------------------------------------------------------
 
@@ -542,4 +726,16 @@ PINNING = {
     "GlibcRandom": "first outputs after srand(1) (1804289383, 846930886, ...) and libc rand()",
     "round_e4m3": "OCP E4M3FN grid KATs (448, 2^-6, 2^-9, ties), brute force over all 256 codes, "
                   "torch float8_e4m3fn casts in range",
+    "lanczos_eigenpairs": "5x5 KAT (bound >= 7.0890, lanczos_test.hpp:44-84); full-length runs on small "
+                          "matrices return the largest-|lambda| eigenpairs of numpy eigh; last-row residual "
+                          "equals ||A x - theta x|| and first-row residual equals |beta_last q0 . x| "
+                          "(identities of an orthonormal basis); diag(1,-2,3,-4,5) picks 5 then -4",
+    "lobpcg": "6x6 KAT top-2 {7.3191, 5.6639} (lobpcg_test.hpp:10-55); top-m of numpy eigh to 1e-8 on "
+              "rand()/RAND_MAX symmetric matrices (lobpcg_test.hpp:57-121); sign=-1 gives the smallest",
+    "composite_FP32_auto_scale_deflate": "diag(1,-2,3,-4,5), k=2 -> diag(1,0,3,0,5) within 1e-3 "
+                                         "(composite_FP32_test.hpp:249-290, k as intended); error vs EVD "
+                                         "projection on generateAndProject inputs",
+    "composite_FP16_auto_scale_deflate": "rel-F error <= 1e-3 on symmetrised rand() matrices and <= 5e-3 "
+                                         "on generateAndProject(radius 10) (composite_FP16_test.hpp)",
+    "eig_FP64_deflate": "equals eig_FP64_psd of the input to rounding (deflation is exact algebra)",
 }

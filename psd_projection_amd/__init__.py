@@ -14,6 +14,9 @@ composite_FP16(_auto_scale)  -> composite_FP16(_auto_scale)(handle, mat, ...)
 composite_FP32_emulated(_auto_scale)
 approximate_two_norm         -> approximate_two_norm(handle, A, n=None, max_iter=20, tol=1e-10) -> (lo, up)
 eig_FP64_psd / eig_FP32_psd  -> eig_FP64_psd(handle, mat, n=None, return_eigenvalues=False)
+compute_eigenpairs           -> compute_eigenpairs(handle, A, k, ...) -> (values, vectors, bound)
+lobpcg                       -> lobpcg(handle, A, m, ...) -> (V, D)
+composite_FP32_auto_scale_deflate, composite_FP16_auto_scale_deflate, eig_FP64_deflate
 
 The reference passes separate cuBLAS and cuSOLVER handles; here one Handle
 (device + stream + workspace arena) takes their place.  The reference prints
@@ -45,7 +48,10 @@ METHODS = {"composite_FP32": METHOD_FP32, "composite_FP16": METHOD_FP16,
 MAX_STEPS = 16
 # psdp_prof_kind_t
 PROF_KINDS = {0: "prod_f32", 1: "prod_f16", 2: "prod_f16x3", 3: "prod_bf16", 4: "prod_bf16x3",
-              5: "prod_bf16x9", 6: "prod_e4m3", 8: "lanczos", 9: "prep"}
+              5: "prod_bf16x9", 6: "prod_e4m3", 8: "lanczos", 9: "prep", 10: "eigenpairs", 11: "lobpcg",
+              12: "rank_update"}
+# psdp_ritz_residual_t
+RITZ_RESIDUAL_FIRST_ROW, RITZ_RESIDUAL_LAST_ROW = 0, 1
 
 
 class PsdpError(RuntimeError):
@@ -114,6 +120,22 @@ _PROTOS = {
                                             ctypes.POINTER(ctypes.c_double)]),
     "psdp_eig_fp64_psd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "psdp_eig_fp32_psd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "psdp_set_eigenpair_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "psdp_compute_eigenpairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                               ctypes.POINTER(ctypes.c_double)]),
+    "psdp_lobpcg": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "psdp_lobpcg_last": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_double)]),
+    "psdp_composite_fp32_auto_scale_deflate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                              ctypes.c_int64, ctypes.c_double, ctypes.c_double]),
+    "psdp_composite_fp16_auto_scale_deflate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                              ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int64,
+                                                              ctypes.c_int, ctypes.c_double]),
+    "psdp_eig_fp64_deflate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.c_int, ctypes.c_double]),
 }
 for _name in ("fp32", "fp32_auto_scale", "fp16", "fp16_auto_scale", "fp32_emulated",
               "fp32_emulated_auto_scale"):
@@ -221,6 +243,18 @@ class Handle:
         lo, up, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _check(lib().psdp_last_bound(self._h, ctypes.byref(lo), ctypes.byref(up), ctypes.byref(it)))
         return lo.value, up.value, it.value
+
+    def set_eigenpair_options(self, residual_row="first", normalize=False):
+        """compute_eigenpairs options: residual row "first" (the reference's)
+        or "last"; normalize divides the kept Ritz vectors by their norms."""
+        m = {"first": RITZ_RESIDUAL_FIRST_ROW, "last": RITZ_RESIDUAL_LAST_ROW}.get(residual_row, residual_row)
+        _check(lib().psdp_set_eigenpair_options(self._h, int(m), 1 if normalize else 0))
+
+    def lobpcg_last(self):
+        """(iterations, ||R||_F) of the last lobpcg run."""
+        it, rn = ctypes.c_int(), ctypes.c_double()
+        _check(lib().psdp_lobpcg_last(self._h, ctypes.byref(it), ctypes.byref(rn)))
+        return it.value, rn.value
 
     def profile(self, on=True):
         """Start (on=True, clears old records) or stop event timing of launches."""
@@ -334,6 +368,85 @@ def eig_FP32_psd(handle, mat, n=None, return_eigenvalues=False):
     return w
 
 
+def _f64_square(A, n):
+    import torch
+    n = _size(A, n)
+    if A.dtype != torch.float64:
+        raise TypeError("expected a float64 matrix")
+    return n
+
+
+def compute_eigenpairs(handle, A, k, n=None, upper_bound_only=False, max_iter=0, tol=1e-10, ortho_tol=1e-5):
+    """src/lanczos.cu:319-547 (lanczos.h:33-43 defaults).  Returns
+    (eigenvalues[:r], eigenvectors[:, :r], bound); eigenvectors are the
+    unnormalised Ritz vectors as columns.  With upper_bound_only, (None, None,
+    bound)."""
+    import torch
+    handle._sync_stream()
+    n = _f64_square(A, n)
+    b = ctypes.c_double()
+    if upper_bound_only:
+        _check(lib().psdp_compute_eigenpairs(handle.ptr, _ptr(A), n, int(k), None, None, None, 1, int(max_iter),
+                                             float(tol), float(ortho_tol), ctypes.byref(b)))
+        return None, None, b.value
+    kk = max(int(k), 1)
+    vals = torch.empty(kk, dtype=torch.float64, device=A.device)
+    vecs = torch.empty((kk, n), dtype=torch.float64, device=A.device)   # row i = column i of the n x k array
+    r = ctypes.c_int64()
+    _check(lib().psdp_compute_eigenpairs(handle.ptr, _ptr(A), n, int(k), ctypes.byref(r), _ptr(vals), _ptr(vecs),
+                                         0, int(max_iter), float(tol), float(ortho_tol), ctypes.byref(b)))
+    return vals[:r.value], vecs[:r.value].T, b.value
+
+
+def lobpcg(handle, A, m, n=None, V=None, D=None, warmstart=False, maxiter=100, tol=1e-8):
+    """src/lobpcg.cu:12-256 (lobpcg.h:16-28 defaults): the m largest
+    eigenpairs.  Returns (V, D): V is n x m with the eigenvectors as columns
+    (column-major storage, i.e. V.T is a contiguous m x n tensor), D the m
+    eigenvalues in descending order.  With warmstart, V (a tensor of that
+    layout) is the orthonormal start block."""
+    import torch
+    handle._sync_stream()
+    n = _f64_square(A, n)
+    if V is None:
+        Vt = torch.empty((m, n), dtype=torch.float64, device=A.device)
+    else:
+        Vt = V.T
+        if tuple(Vt.shape) != (m, n) or not Vt.is_contiguous():
+            raise ValueError("V must be an n x m column-major tensor (V.T contiguous)")
+    if D is None:
+        D = torch.empty(m, dtype=torch.float64, device=A.device)
+    _check(lib().psdp_lobpcg(handle.ptr, _ptr(A), _ptr(Vt), _ptr(D), n, int(m), 1 if warmstart else 0,
+                             int(maxiter), float(tol)))
+    return Vt.T, D
+
+
+def composite_FP32_auto_scale_deflate(handle, mat, n=None, k=40, tol=1e-10, ortho_tol=1e-3):
+    """src/composite_FP32.cu:193-254 (composite_FP32.h:36-45 defaults)."""
+    handle._sync_stream()
+    n = _f64_square(mat, n)
+    _check(lib().psdp_composite_fp32_auto_scale_deflate(handle.ptr, _ptr(mat), n, int(k), float(tol),
+                                                        float(ortho_tol)))
+    return mat
+
+
+def composite_FP16_auto_scale_deflate(handle, mat, n=None, workspace=None, k=40, maxiter=100, tol=1e-10):
+    """src/composite_FP16.cu:276-380 (composite_FP16.h:39-50 defaults)."""
+    handle._sync_stream()
+    n = _f64_square(mat, n)
+    ws, wsb = _ws(workspace)
+    _check(lib().psdp_composite_fp16_auto_scale_deflate(handle.ptr, _ptr(mat), n, ws, wsb, int(k), int(maxiter),
+                                                        float(tol)))
+    return mat
+
+
+def eig_FP64_deflate(handle, mat, n=None, k=40, maxiter=100, tol=1e-10):
+    """src/eig_FP64_psd.cu:87-177 (eig_FP64_psd.h:22-31 defaults)."""
+    handle._sync_stream()
+    n = _f64_square(mat, n)
+    _check(lib().psdp_eig_fp64_deflate(handle.ptr, _ptr(mat), n, int(k), int(maxiter), float(tol)))
+    return mat
+
+
 def project(handle, X, out=None, method="mixed", config: Config | None = None, workspace=None, **overrides):
     """Y = Pi(X) for a float32 or float64 symmetric matrix (X may be `out`)."""
     import torch
@@ -411,4 +524,6 @@ __all__ = [
     "composite_FP32", "composite_FP32_auto_scale", "composite_FP16", "composite_FP16_auto_scale",
     "composite_FP32_emulated", "composite_FP32_emulated_auto_scale", "approximate_two_norm",
     "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "sym_product", "workspace_size",
+    "compute_eigenpairs", "lobpcg", "composite_FP32_auto_scale_deflate", "composite_FP16_auto_scale_deflate",
+    "eig_FP64_deflate",
 ]

@@ -86,12 +86,6 @@ __global__ __launch_bounds__(256) void gemv_batched_kernel(const LanczosJob* __r
 }
 
 // ------------------------------------------------------------------ start
-__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
-  z += 0x9e3779b97f4a7c15ull;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  return z ^ (z >> 31);
-}
 
 // q_i ~ U(0,1] for i < n (zero in the padding), q <- q/||q||, V(:,0) = q,
 // vold = 0 (src/lanczos.cu:69,101-112).  The start vector is a seeded
@@ -102,8 +96,7 @@ __device__ void init_body(LanczosState* st, const LanczosVecs& v, int n, int m, 
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
     double qi = 0.0;
     if (i < n)
-      qi = ((double)(splitmix64(seed ^ (0x632be59bd9b4e019ull * (uint64_t)(i + 1))) >> 11) + 1.0) *
-           (1.0 / 9007199254740992.0);
+      qi = uniform01(seed, (uint64_t)i);
     v.q[i] = qi;
     s += qi * qi;
   }
@@ -450,6 +443,13 @@ hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, 
   hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 2);
   hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin);
+  return hipGetLastError();
+}
+
+// y = A x for a symmetric float64 matrix (also used by the eigenpair Lanczos).
+hipError_t launch_gemv_f64(const double* A, int64_t ld, int m, const double* x, double* y, const int* skip,
+                           hipStream_t s) {
+  launch_gemv(A, ld, m, x, y, skip, s);
   return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "../../include/psd_projection/psdp.h"
+#include "psdp_eigen.h"
+#include "psdp_host.h"
 #include "psdp_internal.h"
 
 namespace psdp {
@@ -54,7 +56,7 @@ using namespace psdp;
 // ---------------------------------------------------------------- errors
 static thread_local std::string g_last_error;
 
-static psdp_status_t fail(psdp_status_t st, const char* fmt, ...) {
+psdp_status_t psdp::fail(psdp_status_t st, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -63,20 +65,6 @@ static psdp_status_t fail(psdp_status_t st, const char* fmt, ...) {
   g_last_error = buf;
   return st;
 }
-
-#define HIPCHK(call)                                                                       \
-  do {                                                                                     \
-    hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess)                                                                  \
-      return fail(PSDP_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                                     \
-  } while (0)
-
-#define STCHK(call)                        \
-  do {                                     \
-    psdp_status_t s_ = (call);             \
-    if (s_ != PSDP_OK) return s_;          \
-  } while (0)
 
 // ---------------------------------------------------------------- tables
 // src/composite_FP32.cu:73-84 (also src/composite_FP32_emulated.cu:78-89)
@@ -213,6 +201,13 @@ struct psdp_handle_st {
   LanczosState* st = nullptr;  // device
   uint64_t seed = 0x5eed5eedULL;
   void* blas = nullptr;  // rocBLAS handle for the EVD drop-ins (created on first use)
+  void* keep = nullptr;  // eigenpairs that outlive the projection of a deflated call
+  size_t keep_bytes = 0;
+  void* pinned = nullptr;  // small pinned host buffer for polled device scalars
+  int ritz_row = PSDP_RITZ_RESIDUAL_FIRST_ROW;
+  int ritz_normalize = 0;  // compute_eigenpairs returns the reference's unnormalised vectors by default
+  int lobpcg_iters = 0;
+  double lobpcg_rnorm = 0.0;
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
   std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, 128-tile tail)
   int cus = 256;                                  // compute units of the device
@@ -540,8 +535,10 @@ static ProdDesc base_desc(const Plan& p) {
 }
 
 // Enqueues the whole projection.
+// fixed_scale: s of the entry points without the bound (1 for the reference's
+// composite_* functions, the deflated matrix's scale for the *_deflate ones).
 static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
-                                        void* Y, int y_f64, const psdp_config_t* cfg) {
+                                        void* Y, int y_f64, const psdp_config_t* cfg, double fixed_scale = 1.0) {
   hipStream_t s = h->stream;
   const Schedule& sch = p.sch;
   // one product: tile size by format, then the launch
@@ -573,7 +570,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     ProfScope ps(h, PSDP_PROF_LANCZOS);
     HIPCHK(launch_lanczos(b.a32, 0, p.ld, p.n, p.npad, iters, tol, h->seed, margin, h->st, b.v, s));
   } else {
-    HIPCHK(launch_set_scale(h->st, 1.0, s));
+    HIPCHK(launch_set_scale(h->st, fixed_scale, s));
   }
   {
     ProfScope ps(h, PSDP_PROF_PREP);
@@ -645,17 +642,9 @@ static psdp_status_t check_flags(psdp_handle_t h) {
   return PSDP_OK;
 }
 
-static psdp_status_t project(psdp_handle_t h, const void* X, int x_f64, void* Y, int y_f64, int64_t n,
-                             const psdp_config_t* cfg, void* ws, size_t ws_bytes) {
-  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
-  if (!X || !Y) return fail(PSDP_ERR_INVALID_ARG, "null matrix pointer");
-  psdp_config_t def;
-  if (!cfg) {
-    psdp_default_config(&def, PSDP_METHOD_MIXED);
-    cfg = &def;
-  }
-  std::lock_guard<std::mutex> lock(h->mu);
-  HIPCHK(hipSetDevice(h->device));
+// Caller holds h->mu and has made h->device current.
+static psdp_status_t project_locked(psdp_handle_t h, const void* X, int x_f64, void* Y, int y_f64, int64_t n,
+                                    const psdp_config_t* cfg, void* ws, size_t ws_bytes, double fixed_scale = 1.0) {
   Plan p;
   STCHK(make_plan(cfg, n, p));
   void* base = ws;
@@ -667,9 +656,23 @@ static psdp_status_t project(psdp_handle_t h, const void* X, int x_f64, void* Y,
     STCHK(arena_reserve(h, p.total, &base));
   }
   Bufs b = carve(p, (char*)base);
-  STCHK(enqueue_projection(h, p, b, X, x_f64, Y, y_f64, cfg));
+  STCHK(enqueue_projection(h, p, b, X, x_f64, Y, y_f64, cfg, fixed_scale));
   if (cfg->synchronize) STCHK(check_flags(h));
   return PSDP_OK;
+}
+
+static psdp_status_t project(psdp_handle_t h, const void* X, int x_f64, void* Y, int y_f64, int64_t n,
+                             const psdp_config_t* cfg, void* ws, size_t ws_bytes) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (!X || !Y) return fail(PSDP_ERR_INVALID_ARG, "null matrix pointer");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  return project_locked(h, X, x_f64, Y, y_f64, n, cfg, ws, ws_bytes);
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -774,6 +777,8 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
     if (h->stage[k]) (void)hipHostFree(h->stage[k]);
   }
   if (h->arena) (void)hipFree(h->arena);
+  if (h->keep) (void)hipFree(h->keep);
+  if (h->pinned) (void)hipHostFree(h->pinned);
   if (h->st) (void)hipFree(h->st);
   eig_destroy_blas(h->blas);
   if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -875,15 +880,9 @@ psdp_status_t psdp_composite_fp32_emulated_auto_scale(psdp_handle_t h, double* m
   return dropin(h, mat, n, ws, ws_bytes, PSDP_METHOD_COMPOSITE_FP32_EMULATED, 1);
 }
 
-psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_t n, double* lo, double* up,
-                                        int64_t max_iter, double tol) {
-  if (!h || !A || !lo || !up) return fail(PSDP_ERR_INVALID_ARG, "null argument");
-  if (n <= 0) return fail(PSDP_ERR_INVALID_ARG, "matrix size must be positive");
-  if (max_iter <= 0) max_iter = 20;
-  if (max_iter > LANCZOS_MAX_ITER)
-    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
-  std::lock_guard<std::mutex> lock(h->mu);
-  HIPCHK(hipSetDevice(h->device));
+// approximate_two_norm on a float64 matrix; caller holds h->mu.
+static psdp_status_t two_norm_locked(psdp_handle_t h, const double* A, int64_t n, int64_t max_iter, double tol,
+                                     double* lo, double* up) {
   const int iters = (int)std::min<int64_t>(max_iter, n);
   const size_t vb = align256((size_t)n * 8);
   void* base = nullptr;
@@ -905,6 +904,19 @@ psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_
   *up = st.up;
   if (st.nonfinite) return fail(PSDP_ERR_NONFINITE, "non-finite value met by the Lanczos bound");
   return PSDP_OK;
+}
+
+psdp_status_t psdp_approximate_two_norm(psdp_handle_t h, const double* A, int64_t n, double* lo, double* up,
+                                        int64_t max_iter, double tol) {
+  if (!h || !A || !lo || !up) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0) return fail(PSDP_ERR_INVALID_ARG, "matrix size must be positive");
+  if (max_iter <= 0) max_iter = 20;
+  if (max_iter > LANCZOS_MAX_ITER)
+    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  ProfScope ps(h, PSDP_PROF_LANCZOS);
+  return two_norm_locked(h, A, n, max_iter, tol, lo, up);
 }
 
 psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float* Y, float* C, int64_t n,
@@ -1273,15 +1285,17 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   return PSDP_OK;
 }
 
-static psdp_status_t eig_dropin(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues, int f64) {
-  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
-  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size %lld", (long long)n);
-  std::lock_guard<std::mutex> lock(h->mu);
-  HIPCHK(hipSetDevice(h->device));
+static psdp_status_t ensure_blas(psdp_handle_t h) {
   if (!h->blas) {
     h->blas = eig_create_blas();
     if (!h->blas) return fail(PSDP_ERR_DEVICE, "rocBLAS handle creation failed");
   }
+  return PSDP_OK;
+}
+
+// Caller holds h->mu.
+static psdp_status_t eig_locked(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues, int f64) {
+  STCHK(ensure_blas(h));
   const size_t wsb = align256(eig_workspace_bytes((int)n, f64));
   void* base = nullptr;
   STCHK(arena_reserve(h, wsb + 256, &base));
@@ -1295,12 +1309,249 @@ static psdp_status_t eig_dropin(psdp_handle_t h, double* mat, int64_t n, double*
   return PSDP_OK;
 }
 
+static psdp_status_t eig_dropin(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues, int f64) {
+  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size %lld", (long long)n);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  return eig_locked(h, mat, n, eigenvalues, f64);
+}
+
 psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
   return eig_dropin(h, mat, n, eigenvalues, 1);
 }
 
 psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
   return eig_dropin(h, mat, n, eigenvalues, 0);
+}
+
+// ---------------------------------------------------------------- eigenpairs and deflation
+static psdp_status_t ensure_pinned(psdp_handle_t h) {
+  if (!h->pinned && hipHostMalloc(&h->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    h->pinned = nullptr;
+    return fail(PSDP_ERR_NOMEM, "pinned host allocation failed");
+  }
+  return PSDP_OK;
+}
+
+// Device memory that survives the projection inside a deflated call.
+static psdp_status_t keep_reserve(psdp_handle_t h, size_t bytes, void** out) {
+  if (bytes > h->keep_bytes) {
+    if (h->keep) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipFree(h->keep));
+      h->keep = nullptr;
+      h->keep_bytes = 0;
+    }
+    if (hipMalloc(&h->keep, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      h->keep = nullptr;
+      return fail(PSDP_ERR_NOMEM, "eigenpair storage of %zu bytes failed", bytes);
+    }
+    h->keep_bytes = bytes;
+  }
+  *out = h->keep;
+  return PSDP_OK;
+}
+
+static psdp_status_t eig_env(psdp_handle_t h, EigEnv* env) {
+  STCHK(ensure_blas(h));
+  STCHK(ensure_pinned(h));
+  env->blas = h->blas;
+  env->stream = h->stream;
+  env->seed = h->seed;
+  env->pinned = h->pinned;
+  return PSDP_OK;
+}
+
+static psdp_status_t eigenpairs_locked(psdp_handle_t h, const double* A, int64_t n, int64_t k, int64_t* r,
+                                       double* evals, double* evecs, bool upper_only, int64_t max_iter,
+                                       double ortho_tol, bool normalize, double* bound) {
+  if (max_iter == 0) max_iter = n;  // src/lanczos.cu:331-332
+  if (upper_only) k = 0;
+  EigEnv env;
+  STCHK(eig_env(h, &env));
+  void* base = nullptr;
+  STCHK(arena_reserve(h, lanczos_pairs_ws_bytes(n, k, max_iter), &base));
+  ProfScope ps(h, PSDP_PROF_EIGENPAIRS);
+  int64_t rr = 0;
+  STCHK(lanczos_pairs(env, A, n, k, max_iter, ortho_tol, h->ritz_row, normalize, upper_only, base, evals, evecs,
+                      &rr, bound));
+  if (r) *r = rr;
+  return PSDP_OK;
+}
+
+static psdp_status_t lobpcg_locked(psdp_handle_t h, const double* A, double sign, double* V, double* D, int64_t n,
+                                   int64_t m, bool warm, int maxiter, double tol) {
+  EigEnv env;
+  STCHK(eig_env(h, &env));
+  void* base = nullptr;
+  STCHK(arena_reserve(h, lobpcg_ws_bytes(n, m), &base));
+  ProfScope ps(h, PSDP_PROF_LOBPCG);
+  return lobpcg_run(env, A, sign, V, D, n, m, warm, maxiter, tol, base, &h->lobpcg_iters, &h->lobpcg_rnorm);
+}
+
+static psdp_status_t update_locked(psdp_handle_t h, double* X, int64_t n, const double* V, const double* lam,
+                                   int64_t r, int64_t split, int mode) {
+  if (r <= 0) return PSDP_OK;
+  EigEnv env;
+  STCHK(eig_env(h, &env));
+  void* base = nullptr;
+  STCHK(arena_reserve(h, align256(8 * (size_t)n * r), &base));
+  ProfScope ps(h, PSDP_PROF_RANK_UPDATE);
+  return rank_update(env, X, n, V, lam, (int)r, (int)split, mode, (double*)base);
+}
+
+// Eigenvectors (n x cols) followed by eigenvalues (cols) in the keep area.
+static psdp_status_t keep_pairs(psdp_handle_t h, int64_t n, int64_t cols, double** V, double** lam) {
+  cols = std::max<int64_t>(cols, 1);
+  void* base = nullptr;
+  const size_t vb = align256(8 * (size_t)n * cols);
+  STCHK(keep_reserve(h, vb + align256(8 * (size_t)cols), &base));
+  *V = (double*)base;
+  *lam = (double*)((char*)base + vb);
+  return PSDP_OK;
+}
+
+static bool size_ok(int64_t n) { return n > 0 && n <= 65536; }
+
+psdp_status_t psdp_set_eigenpair_options(psdp_handle_t h, int residual_row, int normalize_vectors) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (residual_row != PSDP_RITZ_RESIDUAL_FIRST_ROW && residual_row != PSDP_RITZ_RESIDUAL_LAST_ROW)
+    return fail(PSDP_ERR_INVALID_ARG, "unknown Ritz residual row %d", residual_row);
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->ritz_row = residual_row;
+  h->ritz_normalize = normalize_vectors != 0;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_compute_eigenpairs(psdp_handle_t h, const double* A, int64_t n, int64_t k, int64_t* r,
+                                      double* eigenvalues, double* eigenvectors, int upper_bound_only,
+                                      int64_t max_iter, double tol, double ortho_tol, double* upper_bound) {
+  (void)tol;  // the reference filters with ortho_tol only (src/lanczos.cu:437)
+  if (!h || !A || !upper_bound) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (!size_ok(n)) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size %lld", (long long)n);
+  if (k < 0 || max_iter < 0) return fail(PSDP_ERR_INVALID_ARG, "negative k or max_iter");
+  if (!upper_bound_only && (!r || (k > 0 && (!eigenvalues || !eigenvectors))))
+    return fail(PSDP_ERR_INVALID_ARG, "null output array");
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  return eigenpairs_locked(h, A, n, k, upper_bound_only ? nullptr : r, eigenvalues, eigenvectors,
+                           upper_bound_only != 0, max_iter, ortho_tol, h->ritz_normalize != 0, upper_bound);
+}
+
+psdp_status_t psdp_lobpcg(psdp_handle_t h, const double* A, double* V, double* D, int64_t n, int64_t m,
+                          int warmstart, int maxiter, double tol) {
+  if (!h || !A || !V || !D) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (!size_ok(n) || m <= 0 || 3 * m > n)
+    return fail(PSDP_ERR_INVALID_ARG, "lobpcg needs 0 < 3 m <= n (n = %lld, m = %lld)", (long long)n,
+                (long long)m);
+  if (maxiter < 0) return fail(PSDP_ERR_INVALID_ARG, "negative maxiter");
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  return lobpcg_locked(h, A, 1.0, V, D, n, m, warmstart != 0, maxiter, tol);
+}
+
+psdp_status_t psdp_lobpcg_last(psdp_handle_t h, int* iterations, double* residual_norm) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (iterations) *iterations = h->lobpcg_iters;
+  if (residual_norm) *residual_norm = h->lobpcg_rnorm;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_composite_fp32_auto_scale_deflate(psdp_handle_t h, double* mat, int64_t n, int64_t k,
+                                                     double tol, double ortho_tol) {
+  (void)tol;
+  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (!size_ok(n) || k < 0) return fail(PSDP_ERR_INVALID_ARG, "bad size (n = %lld, k = %lld)", (long long)n,
+                                        (long long)k);
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  double *V = nullptr, *lam = nullptr;
+  STCHK(keep_pairs(h, n, k, &V, &lam));
+  int64_t r = 0;
+  double bound = 0.0;
+  // src/composite_FP32.cu:204-225: the k pairs of largest |lambda|, removed;
+  // the kept Ritz vectors are normalised (the reference's are not: with a
+  // basis that lost orthogonality its x x^T removes theta ||x||^2, which
+  // leaves (||x||^2 - 1) theta behind in the result)
+  if (k > 0) STCHK(eigenpairs_locked(h, mat, n, k, &r, lam, V, false, 0, ortho_tol, true, &bound));
+  STCHK(update_locked(h, mat, n, V, lam, r, r, RANK_REMOVE));
+  // :227-235: bound of the deflated matrix, s = 1.1 up + 1e-5
+  double up = 0.0;
+  STCHK(eigenpairs_locked(h, mat, n, 0, nullptr, nullptr, nullptr, true, 100, 1e-5, false, &up));
+  const double scale = up > 0.0 ? 1.1 * up + 1e-5 : 1.0;
+  // :237-241: s composite_FP32(X / s)
+  psdp_config_t cfg;
+  psdp_default_config(&cfg, PSDP_METHOD_COMPOSITE_FP32);
+  cfg.auto_scale = 0;
+  STCHK(project_locked(h, mat, 1, mat, 1, n, &cfg, nullptr, 0, scale));
+  // :243-250: the positive ones added back
+  STCHK(update_locked(h, mat, n, V, lam, r, r, RANK_ADD_POSITIVE));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PSDP_OK;
+}
+
+// LOBPCG deflation shared by the FP16 and FP64 deflated projections
+// (src/composite_FP16.cu:300-333, src/eig_FP64_psd.cu:104-137): the k largest
+// pairs are removed, then the k smallest of the remainder (as the largest of
+// its negation).  V: n x 2k, lam: 2k (largest, then those of the negation).
+static psdp_status_t lobpcg_deflate_locked(psdp_handle_t h, double* mat, int64_t n, int64_t k, int maxiter,
+                                           double tol, double* V, double* lam) {
+  if (k == 0) return PSDP_OK;
+  STCHK(lobpcg_locked(h, mat, 1.0, V, lam, n, k, false, maxiter, tol));
+  STCHK(update_locked(h, mat, n, V, lam, k, k, RANK_REMOVE));
+  STCHK(lobpcg_locked(h, mat, -1.0, V + n * k, lam + k, n, k, false, maxiter, tol));
+  STCHK(update_locked(h, mat, n, V + n * k, lam + k, k, 0, RANK_REMOVE));
+  return PSDP_OK;
+}
+
+static psdp_status_t lobpcg_deflate_args(psdp_handle_t h, double* mat, int64_t n, int64_t k, int maxiter) {
+  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (!size_ok(n) || k < 0 || 3 * k > n)
+    return fail(PSDP_ERR_INVALID_ARG, "deflation by lobpcg needs 0 <= 3 k <= n (n = %lld, k = %lld)",
+                (long long)n, (long long)k);
+  if (maxiter < 0) return fail(PSDP_ERR_INVALID_ARG, "negative maxiter");
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_composite_fp16_auto_scale_deflate(psdp_handle_t h, double* mat, int64_t n, void* ws,
+                                                     size_t ws_bytes, int64_t k, int maxiter, double tol) {
+  STCHK(lobpcg_deflate_args(h, mat, n, k, maxiter));
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  double *V = nullptr, *lam = nullptr;
+  STCHK(keep_pairs(h, n, 2 * k, &V, &lam));
+  STCHK(lobpcg_deflate_locked(h, mat, n, k, maxiter, tol, V, lam));
+  // src/composite_FP16.cu:335-346: Lanczos bound, s = 1.1 up + 1e-5
+  double lo = 0.0, up = 0.0;
+  {
+    ProfScope ps(h, PSDP_PROF_LANCZOS);
+    STCHK(two_norm_locked(h, mat, n, 20, 1e-10, &lo, &up));
+  }
+  const double scale = up > 0.0 ? 1.1 * up + 1e-5 : 1.0;
+  psdp_config_t cfg;
+  psdp_default_config(&cfg, PSDP_METHOD_COMPOSITE_FP16);
+  cfg.auto_scale = 0;
+  STCHK(project_locked(h, mat, 1, mat, 1, n, &cfg, ws, ws_bytes, scale));
+  // :351-370
+  STCHK(update_locked(h, mat, n, V, lam, 2 * k, k, RANK_ADD_POSITIVE));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_eig_fp64_deflate(psdp_handle_t h, double* mat, int64_t n, int64_t k, int maxiter, double tol) {
+  STCHK(lobpcg_deflate_args(h, mat, n, k, maxiter));
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  double *V = nullptr, *lam = nullptr;
+  STCHK(keep_pairs(h, n, 2 * k, &V, &lam));
+  STCHK(lobpcg_deflate_locked(h, mat, n, k, maxiter, tol, V, lam));
+  STCHK(eig_locked(h, mat, n, nullptr, 1));  // src/eig_FP64_psd.cu:139-140
+  STCHK(update_locked(h, mat, n, V, lam, 2 * k, k, RANK_ADD_POSITIVE));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PSDP_OK;
 }
 
 }  // extern "C"

@@ -99,6 +99,20 @@ __device__ __forceinline__ void store_operand(int fmt, float v, int64_t idx, flo
 }
 
 // ---------------------------------------------------------------------------
+// Seeded counter-based uniform stream: element i of stream `seed` is
+// (splitmix64(seed ^ C (i + 1)) >> 11 + 1) 2^-53, a value in (0, 1].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double uniform01(uint64_t seed, uint64_t i) {
+  return ((double)(splitmix64(seed ^ (0x632be59bd9b4e019ull * (i + 1))) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
 // Wave / block reductions (wave64).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {

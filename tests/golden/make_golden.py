@@ -7,6 +7,10 @@ mathematics (numpy/LAPACK eigen-decompositions):
   * composite_FP32_test.hpp:258-276 diag(1,-2,3,-4,5) -> diag(1,0,3,0,5) (tol 1e-3)
   * lanczos_test.hpp:20-31          5x5 matrix, ||A||_2 = 7.0890 (lo <= . <= up)
   * lobpcg_test.hpp:23-42           6x6 matrix, top-2 eigenvalues 7.3191, 5.6639
+  * lanczos_test.hpp:44-84          5x5 matrix, compute_eigenpairs(k=3) bound >= 7.0890
+  * composite_FP32_test.hpp:249-290 the diag(1,-2,3,-4,5) case through the deflated
+                                    projection with k = 2 (the test binds 2 to tol and
+                                    0 to k by position; k = 2 is what it means)
 transfer_*.npz holds the scalar transfer function g(x) of both coefficient
 tables on a grid (computed by the oracle; regression data for the GPU tests).
 chain_small.npz holds small FP64 chain inputs/outputs (regression data).
@@ -56,6 +60,16 @@ def main():
         "source": "test/lobpcg_test.hpp:10-55",
         "n": 6, "input_colmajor": L6, "expected_top2_reference": [7.3191, 5.6639],
         "eigenvalues": np.linalg.eigvalsh(M6).tolist(), "tol": 1e-4,
+    }
+    kats["eigenpairs_5x5"] = {
+        "source": "test/lanczos_test.hpp:44-84",
+        "n": 5, "input_colmajor": L5, "k": 3, "two_norm_reference": 7.0890,
+        "eigenvalues": np.linalg.eigvalsh(M5).tolist(),
+    }
+    kats["simple5_deflate"] = {
+        "source": "test/composite_FP32_test.hpp:249-290",
+        "n": 5, "diagonal": d5, "expected_diagonal": [max(x, 0.0) for x in d5], "k": 2, "tol": 1e-3,
+        "entry": "composite_FP32_auto_scale_deflate",
     }
     (HERE / "reference_kats.json").write_text(json.dumps(kats, indent=1))
 

@@ -22,6 +22,13 @@ int main(int argc, char**) {
     psd_projection::approximate_two_norm(h, h, m, 10, &lo, &up);
     psd_projection::eig_FP64_psd(h, h, m, 10);
     psd_projection::eig_FP32_psd(h, h, m, 10);
+    size_t r = 0;
+    psd_projection::composite_FP32_auto_scale_deflate(h, h, m, 10);
+    psd_projection::composite_FP16_auto_scale_deflate(h, h, m, 10);
+    psd_projection::eig_FP64_deflate(h, h, m, 10);
+    double b = psd_projection::compute_eigenpairs(h, h, m, 10, 3, &r, m, m);
+    psd_projection::lobpcg(h, h, m, m, m, 10, 2);
+    (void)b;
   }
   psdp_config_t cfg;
   size_t b = 0;

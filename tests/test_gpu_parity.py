@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from oracle import psdp_oracle as O
+from _streams import start_vector
 
 torch = pytest.importorskip("torch")
 import psd_projection_amd as P  # noqa: E402
@@ -34,22 +35,6 @@ def to_dev(M, dtype=torch.float64):
 
 def from_dev(T):
     return T.detach().cpu().numpy().astype(np.float64).T.copy()
-
-
-def splitmix64(z):
-    m = (1 << 64) - 1
-    z = (z + 0x9E3779B97F4A7C15) & m
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
-    return z ^ (z >> 31)
-
-
-def start_vector(seed, n):
-    """The library's Lanczos start vector: q_i ~ U(0,1] from a splitmix64
-    counter stream (lanczos.hip, lanczos_init_kernel)."""
-    m = (1 << 64) - 1
-    return np.array([((splitmix64(seed ^ ((0x632BE59BD9B4E019 * (i + 1)) & m)) >> 11) + 1) * 2.0 ** -53
-                     for i in range(n)])
 
 
 # ------------------------------------------------------------------ products
