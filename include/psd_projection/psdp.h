@@ -158,7 +158,8 @@ typedef enum psdp_prof_kind {
   PSDP_PROF_EIGENPAIRS = 10, /* psdp_compute_eigenpairs (whole call)                */
   PSDP_PROF_LOBPCG = 11,  /* psdp_lobpcg (whole call)                                */
   PSDP_PROF_RANK_UPDATE = 12, /* X += V diag(c) V^T of the deflated projections      */
-  PSDP_PROF_KINDS = 13
+  PSDP_PROF_GRAPH = 13,   /* a replayed batched graph (whole device sequence)       */
+  PSDP_PROF_KINDS = 14
 } psdp_prof_kind_t;
 /* on != 0 starts recording (and clears old records); 0 stops. */
 psdp_status_t psdp_profile_enable(psdp_handle_t h, int on);
@@ -173,6 +174,30 @@ typedef struct psdp_block {
 } psdp_block_t;
 psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
                                        const psdp_config_t* cfg);
+/* Tile classes of the batched call: blocks with n >= big_block_n use
+ * 128 x 128 tiles, the rest 64 x 64 (0: the built-in default); concurrent != 0
+ * runs the two classes of each product on two streams. */
+psdp_status_t psdp_set_batched_tiling(psdp_handle_t h, int big_block_n, int concurrent);
+/* on != 0: the device-side sequence of a batched call (everything after the
+ * descriptor upload) is captured once into a HIP graph and replayed by later
+ * calls with the same block sizes, configuration, tiling and workspace. */
+psdp_status_t psdp_set_batched_graph(psdp_handle_t h, int on);
+/* Autotuner.  psdp_autotune times every tile size the product kernels offer
+ * for a projection of size n with cfg (each operand format of its schedule,
+ * on a synthetic symmetric matrix) and keeps the fastest in the handle; later
+ * projections of that size use it.  psdp_tuned_tile reports the choice (tile
+ * edge in elements, 0 if none).  psdp_autotune_batched runs the batched
+ * projection of these blocks under each tile-class setting and keeps the
+ * fastest (as psdp_set_batched_tiling would); it writes the blocks' outputs,
+ * which every setting computes bitwise alike.  All block until done. */
+psdp_status_t psdp_autotune(psdp_handle_t h, int64_t n, const psdp_config_t* cfg);
+psdp_status_t psdp_tuned_tile(psdp_handle_t h, int64_t n, int format, int average, int* tile);
+/* Sets the tile edge (64, 128 or 256; 0 clears) of products of size n in
+ * `format` as the autotuner would; edges above the size's largest tile (256
+ * from n = 2048 for 16-bit formats, 128 above 1024, else 64) are ignored. */
+psdp_status_t psdp_set_tuned_tile(psdp_handle_t h, int64_t n, int format, int average, int tile);
+psdp_status_t psdp_autotune_batched(psdp_handle_t h, const psdp_block_t* blocks, int count,
+                                    const psdp_config_t* cfg);
 /* Workspace the batched call needs (host only: also builds and checks every
  * launch descriptor of the call without touching a device). */
 psdp_status_t psdp_workspace_size_batched(const psdp_block_t* blocks, int count, const psdp_config_t* cfg,

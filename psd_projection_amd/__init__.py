@@ -49,7 +49,7 @@ MAX_STEPS = 16
 # psdp_prof_kind_t
 PROF_KINDS = {0: "prod_f32", 1: "prod_f16", 2: "prod_f16x3", 3: "prod_bf16", 4: "prod_bf16x3",
               5: "prod_bf16x9", 6: "prod_e4m3", 8: "lanczos", 9: "prep", 10: "eigenpairs", 11: "lobpcg",
-              12: "rank_update"}
+              12: "rank_update", 13: "graph"}
 # psdp_ritz_residual_t
 RITZ_RESIDUAL_FIRST_ROW, RITZ_RESIDUAL_LAST_ROW = 0, 1
 
@@ -121,6 +121,14 @@ _PROTOS = {
     "psdp_eig_fp64_psd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "psdp_eig_fp32_psd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "psdp_set_eigenpair_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "psdp_set_batched_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "psdp_set_batched_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "psdp_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(Config)]),
+    "psdp_tuned_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int)]),
+    "psdp_set_tuned_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "psdp_autotune_batched": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Block), ctypes.c_int,
+                                             ctypes.POINTER(Config)]),
     "psdp_compute_eigenpairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
@@ -249,6 +257,47 @@ class Handle:
         or "last"; normalize divides the kept Ritz vectors by their norms."""
         m = {"first": RITZ_RESIDUAL_FIRST_ROW, "last": RITZ_RESIDUAL_LAST_ROW}.get(residual_row, residual_row)
         _check(lib().psdp_set_eigenpair_options(self._h, int(m), 1 if normalize else 0))
+
+    def set_batched_tiling(self, big_block_n=0, concurrent=False):
+        """Blocks with n >= big_block_n run on 128 x 128 tiles in batched calls
+        (0: default); concurrent runs the two tile classes on two streams."""
+        _check(lib().psdp_set_batched_tiling(self._h, int(big_block_n), 1 if concurrent else 0))
+
+    def set_batched_graph(self, on=True):
+        """Capture the device-side sequence of batched calls in a HIP graph and
+        replay it for calls with the same plan."""
+        _check(lib().psdp_set_batched_graph(self._h, 1 if on else 0))
+
+    def autotune(self, n, method="mixed", config: Config | None = None, **overrides):
+        """Times the product tile sizes for projections of size n and keeps the
+        fastest; returns {(format, averaged): tile edge}."""
+        cfg = config if config is not None else default_config(method, **overrides)
+        _check(lib().psdp_autotune(self._h, int(n), ctypes.byref(cfg)))
+        out = {}
+        fmts = [cfg.step_format[i] for i in range(MAX_STEPS)] + [cfg.final_format]
+        for f in sorted(set(FORMATS.values())):
+            for avg in (0, 1):
+                t = ctypes.c_int()
+                _check(lib().psdp_tuned_tile(self._h, int(n), f, avg, ctypes.byref(t)))
+                if t.value:
+                    out[(f, avg)] = t.value
+        return out
+
+    def set_tile(self, n, fmt, tile, average=False):
+        """Tile edge (64/128/256, 0 clears) of products of size n in format fmt."""
+        f = FORMATS[fmt] if isinstance(fmt, str) else int(fmt)
+        _check(lib().psdp_set_tuned_tile(self._h, int(n), f, 1 if average else 0, int(tile)))
+
+    def autotune_batched(self, blocks, outs, method="mixed", config: Config | None = None, **overrides):
+        """Picks the batched tile classes by timing these blocks (writes outs)."""
+        self._sync_stream()
+        arr = (Block * len(blocks))()
+        for i, (x, y) in enumerate(zip(blocks, outs)):
+            arr[i].X = _ptr(x)
+            arr[i].Y = _ptr(y)
+            arr[i].n = _size(x, None)
+        cfg = config if config is not None else default_config(method, **overrides)
+        _check(lib().psdp_autotune_batched(self._h, arr, len(blocks), ctypes.byref(cfg)))
 
     def lobpcg_last(self):
         """(iterations, ||R||_F) of the last lobpcg run."""

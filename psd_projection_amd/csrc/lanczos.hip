@@ -27,6 +27,7 @@ namespace psdp {
 
 constexpr int GEMV_RPW = 4;                    // rows per wave
 constexpr int GEMV_ROWS_PER_BLOCK = 4 * GEMV_RPW;
+static_assert(GEMV_ROWS_PER_BLOCK == GEMV_BATCH_ROWS, "batched GEMV row groups");
 
 // ------------------------------------------------------------------ GEMV
 // y[r] = sum_k A[r*ld + k] x[k] for the rows of this workgroup (A symmetric:
@@ -40,17 +41,20 @@ __device__ __forceinline__ void gemv_body(const TA* __restrict__ A, int64_t ld, 
   double acc[GEMV_RPW];
 #pragma unroll
   for (int r = 0; r < GEMV_RPW; ++r) acc[r] = 0.0;
+  const auto Ag = gbl(A);
+  const auto xg = gbl(x);
   for (int k = lane * VEC; k < m; k += 64 * VEC) {
     double xv[VEC];
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) xv[e] = x[k + e];
+    for (int e = 0; e < VEC; ++e) xv[e] = xg[k + e];
 #pragma unroll
     for (int r = 0; r < GEMV_RPW; ++r) {
       if (r0 + r < m) {
-        const TA* row = A + (int64_t)(r0 + r) * ld + k;
+        const auto row = Ag + (int64_t)(r0 + r) * ld + k;
         TA av[VEC];
         if constexpr (VEC * sizeof(TA) == 16) {
-          *(uint4*)av = *(const uint4*)row;
+          typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+          *(u32x4v*)av = *(const __attribute__((address_space(1))) u32x4v*)row;
         } else {
 #pragma unroll
           for (int e = 0; e < VEC; ++e) av[e] = row[e];
@@ -63,7 +67,7 @@ __device__ __forceinline__ void gemv_body(const TA* __restrict__ A, int64_t ld, 
 #pragma unroll
   for (int r = 0; r < GEMV_RPW; ++r) {
     const double s = wave_sum(acc[r]);
-    if (lane == 0 && r0 + r < m) y[r0 + r] = s;
+    if (lane == 0 && r0 + r < m) gbl(y)[r0 + r] = s;
   }
 }
 
@@ -75,14 +79,17 @@ __global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A,
   gemv_body<TA, VEC>(A, ld, m, x, y, blockIdx.x);
 }
 
-// which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y (skip unless Ritz pair)
-__global__ __launch_bounds__(256) void gemv_batched_kernel(const LanczosJob* __restrict__ jobs, int which) {
-  const LanczosJob& J = jobs[blockIdx.y];
+// which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y (skip unless Ritz pair).
+// Workgroup g takes the GEMV_ROWS_PER_BLOCK rows rows[g].y of matrix rows[g].x
+// (a flat list over every matrix's row groups, so no workgroup idles).
+__global__ __launch_bounds__(256) void gemv_batched_kernel(const LanczosJob* __restrict__ jobs,
+                                                           const int2* __restrict__ rows, int which) {
+  const int2 rg = rows[blockIdx.x];
+  const LanczosJob& J = jobs[rg.x];
   if (which < 2 ? J.st->done : J.st->skip_ritz) return;
-  if ((int)blockIdx.x * GEMV_ROWS_PER_BLOCK >= J.m) return;
   const double* x = which == 0 ? J.v.q : (which == 1 ? J.v.w1 : J.v.y);
   double* y = which == 0 ? J.v.w1 : (which == 1 ? J.v.w : J.v.ay);
-  gemv_body<float, 4>(J.A, J.ld, J.m, x, y, blockIdx.x);
+  gemv_body<float, 4>(J.A, J.ld, J.m, x, y, rg.y);
 }
 
 // ------------------------------------------------------------------ start
@@ -430,18 +437,20 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
 
 // Batched bound over `count` float32 padded matrices (jobs in device memory;
 // every job's m is a multiple of 4 and at most max_m).
+// rows: one int2 {matrix, row group} per GEMV_ROWS_PER_BLOCK rows of every
+// matrix (nrows entries).
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, hipStream_t s) {
-  const dim3 ggrid((max_m + GEMV_ROWS_PER_BLOCK - 1) / GEMV_ROWS_PER_BLOCK, count);
+                                  uint64_t seed, double margin, const int2* rows, int nrows, hipStream_t s) {
+  const dim3 ggrid(nrows);
   hipLaunchKernelGGL(lanczos_init_batched, dim3(count), dim3(1024), 0, s, jobs, seed);
   for (int k = 0; k < max_iter; ++k) {
-    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 0);
-    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 1);
+    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 0);
+    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 1);
     hipLaunchKernelGGL(lanczos_step_batched, dim3(count), dim3(1024), 0, s, jobs, k, tol);
   }
   hipLaunchKernelGGL(lanczos_ritz_batched, dim3(count), dim3(64), 0, s, jobs);
   hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
-  hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, 2);
+  hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 2);
   hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin);
   return hipGetLastError();
 }

@@ -109,13 +109,13 @@ __global__ __launch_bounds__(256) void scale_split_kernel(SplitArgs s) {
   }
 }
 
-// Batched form (float32 inputs): blockIdx.z selects the matrix; tiles beyond
-// that matrix's padded size exit at once.
-__global__ __launch_bounds__(256) void sym_pad_batched_kernel(const PadJob* __restrict__ jobs) {
-  const PadJob& J = jobs[blockIdx.z];
-  const int T = J.npad / 64;
-  const int bi = blockIdx.y, bj = blockIdx.x;
-  if (bi >= T || bj >= T || bi > bj) return;
+// Batched form (float32 inputs): workgroup g takes the 64 x 64 tile pair
+// tiles[g] (matrix, bi <= bj) from a flat list over every matrix.
+__global__ __launch_bounds__(256) void sym_pad_batched_kernel(const PadJob* __restrict__ jobs,
+                                                              const TileRef* __restrict__ tiles) {
+  const TileRef tr = tiles[blockIdx.x];
+  const PadJob& J = jobs[tr.mat];
+  const int bi = tr.bi, bj = tr.bj;
   __shared__ float s1[64][65];
   __shared__ float s2[64][65];
   const int t = threadIdx.x;
@@ -123,49 +123,118 @@ __global__ __launch_bounds__(256) void sym_pad_batched_kernel(const PadJob* __re
   const int i0 = bi * 64, j0 = bj * 64;
   const int n = J.n;
   const int64_t ld = J.npad;
+  const auto Xg = gbl(J.X);
+  const auto Mg = gbl(J.M);
 #pragma unroll 4
   for (int c = t >> 6; c < 64; c += 4) {
     const int gi = i0 + r, gj = j0 + c;
-    s1[c][r] = (gi < n && gj < n) ? J.X[(int64_t)gi + (int64_t)gj * n] : 0.0f;
+    s1[c][r] = (gi < n && gj < n) ? Xg[(int64_t)gi + (int64_t)gj * n] : 0.0f;
     const int hi = j0 + r, hj = i0 + c;
-    s2[c][r] = (hi < n && hj < n) ? J.X[(int64_t)hi + (int64_t)hj * n] : 0.0f;
+    s2[c][r] = (hi < n && hj < n) ? Xg[(int64_t)hi + (int64_t)hj * n] : 0.0f;
   }
   __syncthreads();
 #pragma unroll 4
   for (int a = t >> 6; a < 64; a += 4) {
     const int b = r;
-    J.M[(int64_t)(i0 + a) * ld + j0 + b] = 0.5f * (s1[b][a] + s2[a][b]);
-    if (bi != bj) J.M[(int64_t)(j0 + a) * ld + i0 + b] = 0.5f * (s2[b][a] + s1[a][b]);
+    Mg[(int64_t)(i0 + a) * ld + j0 + b] = 0.5f * (s1[b][a] + s2[a][b]);
+    if (bi != bj) Mg[(int64_t)(j0 + a) * ld + i0 + b] = 0.5f * (s2[b][a] + s1[a][b]);
   }
 }
 
-__global__ __launch_bounds__(256) void scale_split_batched_kernel(const SplitJob* __restrict__ jobs, int fmt_a,
+// Stores four consecutive values (index i, a multiple of 4) in format fmt
+// with one vector store per piece.
+__device__ __forceinline__ void store_operand4(int fmt, const float v[4], int64_t i, float* f32,
+                                               uint16_t* const* pieces) {
+  if (fmt == FMT_F32) {
+    *(__attribute__((address_space(1))) f32x4*)gbl(f32 + i) = f32x4{v[0], v[1], v[2], v[3]};
+    return;
+  }
+  uint16_t p[4][3];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) split_value(fmt, v[e], p[e]);
+  const int np = fmt_pieces(fmt);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (q >= np) break;
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 w = {(uint32_t)p[0][q] | ((uint32_t)p[1][q] << 16), (uint32_t)p[2][q] | ((uint32_t)p[3][q] << 16)};
+    *(__attribute__((address_space(1))) u32x2*)gbl(pieces[q] + i) = w;
+  }
+}
+
+// Workgroup g scales and splits the SPLIT_CHUNK values chunks[g].y of matrix
+// chunks[g].x (every matrix holds a multiple of 64 x 64 values).
+__global__ __launch_bounds__(256) void scale_split_batched_kernel(const SplitJob* __restrict__ jobs,
+                                                                  const int2* __restrict__ chunks, int fmt_a,
                                                                   int fmt_x) {
-  const SplitJob& J = jobs[blockIdx.y];
+  const int2 ck = chunks[blockIdx.x];
+  const SplitJob& J = jobs[ck.x];
   const double inv = J.inv ? *J.inv : 1.0;
   uint16_t* ap[3] = {J.ap[0], J.ap[1], J.ap[2]};
   uint16_t* xp[3] = {J.xp[0], J.xp[1], J.xp[2]};
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < J.count; i += stride) {
-    const float v = (float)((double)J.a32[i] * inv);
-    J.a32[i] = v;
-    if (fmt_a != FMT_F32) store_operand(fmt_a, v, i, nullptr, ap);
-    store_operand(fmt_x, v, i, J.x32, xp);
+  const int64_t base = (int64_t)ck.y * SPLIT_CHUNK;
+#pragma unroll
+  for (int r = 0; r < SPLIT_CHUNK / 1024; ++r) {
+    const int64_t i = base + (int64_t)r * 1024 + 4 * threadIdx.x;
+    const f32x4 a = *(const __attribute__((address_space(1))) f32x4*)gbl(J.a32 + i);
+    const float v[4] = {(float)((double)a[0] * inv), (float)((double)a[1] * inv), (float)((double)a[2] * inv),
+                        (float)((double)a[3] * inv)};
+    *(__attribute__((address_space(1))) f32x4*)gbl(J.a32 + i) = f32x4{v[0], v[1], v[2], v[3]};
+    if (fmt_a != FMT_F32) store_operand4(fmt_a, v, i, nullptr, ap);
+    store_operand4(fmt_x, v, i, J.x32, xp);
   }
 }
 
-hipError_t launch_sym_pad_batched(const PadJob* jobs, int count, int max_npad, hipStream_t st) {
-  const int T = max_npad / 64;
-  hipLaunchKernelGGL(sym_pad_batched_kernel, dim3(T, T, count), dim3(256), 0, st, jobs);
+// Synthetic symmetric matrix for the autotuner: M(i, j) = (2 u - 1)/sqrt(n)
+// with u from the seeded stream at index min(i,j) * n + max(i,j) for
+// i, j < n, zero elsewhere (columns up to ld).
+__global__ __launch_bounds__(256) void fill_sym_kernel(float* __restrict__ M, int64_t ld, int n, int npad,
+                                                       uint64_t seed) {
+  const int64_t total = (int64_t)npad * ld;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const float sc = 1.0f / sqrtf((float)n);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int i = (int)(e / ld), j = (int)(e % ld);
+    float v = 0.0f;
+    if (i < n && j < n) {
+      const int a = i < j ? i : j, b = i < j ? j : i;
+      v = (2.0f * (float)uniform01(seed, (uint64_t)a * n + b) - 1.0f) * sc;
+    }
+    M[e] = v;
+  }
+}
+
+// dst = src for `words` 8-byte words (device to device, as a kernel so that a
+// captured graph holds no copy node).
+__global__ void copy_words_kernel(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src, int64_t words) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(void* dst, const void* src, int64_t words, hipStream_t st) {
+  int64_t blocks = (words + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(copy_words_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint64_t*)dst,
+                     (const uint64_t*)src, words);
   return hipGetLastError();
 }
 
-hipError_t launch_scale_split_batched(const SplitJob* jobs, int count, int64_t max_count, int fmt_a, int fmt_x,
+hipError_t launch_fill_sym(float* M, int64_t ld, int n, int npad, uint64_t seed, hipStream_t st) {
+  int64_t blocks = ((int64_t)npad * ld + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_sym_kernel, dim3((unsigned)blocks), dim3(256), 0, st, M, ld, n, npad, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_pad_batched(const PadJob* jobs, const TileRef* tiles, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(sym_pad_batched_kernel, dim3(ntiles), dim3(256), 0, st, jobs, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_split_batched(const SplitJob* jobs, const int2* chunks, int nchunks, int fmt_a, int fmt_x,
                                       hipStream_t st) {
-  int64_t blocks = (max_count + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(scale_split_batched_kernel, dim3((unsigned)blocks, count), dim3(256), 0, st, jobs, fmt_a,
-                     fmt_x);
+  hipLaunchKernelGGL(scale_split_batched_kernel, dim3(nchunks), dim3(256), 0, st, jobs, chunks, fmt_a, fmt_x);
   return hipGetLastError();
 }
 

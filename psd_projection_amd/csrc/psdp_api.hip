@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -40,10 +41,12 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
                           uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
                           hipStream_t s);
 hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s);
+hipError_t launch_fill_sym(float* M, int64_t ld, int n, int npad, uint64_t seed, hipStream_t st);
+hipError_t launch_copy_words(void* dst, const void* src, int64_t words, hipStream_t st);
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, hipStream_t s);
-hipError_t launch_sym_pad_batched(const PadJob* jobs, int count, int max_npad, hipStream_t st);
-hipError_t launch_scale_split_batched(const SplitJob* jobs, int count, int64_t max_count, int fmt_a, int fmt_x,
+                                  uint64_t seed, double margin, const int2* rows, int nrows, hipStream_t s);
+hipError_t launch_sym_pad_batched(const PadJob* jobs, const TileRef* tiles, int ntiles, hipStream_t st);
+hipError_t launch_scale_split_batched(const SplitJob* jobs, const int2* chunks, int nchunks, int fmt_a, int fmt_x,
                                       hipStream_t st);
 size_t eig_workspace_bytes(int n, int f64);
 int eig_project(void* rb_handle, hipStream_t s, double* X, int n, int f64, void* ws, int* info_dev, double* eig);
@@ -208,6 +211,19 @@ struct psdp_handle_st {
   int ritz_normalize = 0;  // compute_eigenpairs returns the reference's unnormalised vectors by default
   int lobpcg_iters = 0;
   double lobpcg_rnorm = 0.0;
+  // batched tile classes: blocks with n >= batch_big_n on 128 x 128 tiles;
+  // with batch_concurrent the two classes of a stage run on two streams
+  int batch_big_n = 0;  // 0: BIG_BLOCK_N
+  std::map<std::tuple<int, int, int>, int> tuned;  // (n, format, averaged) -> tile sub (psdp_autotune)
+  int batch_concurrent = 0;
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // batched calls replayed from a captured HIP graph (psdp_set_batched_graph):
+  // the graph of the last call and the key it was captured for
+  int batch_graph = 0;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t bgraph = nullptr;
+  std::vector<int64_t> bgraph_key;
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
   std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, 128-tile tail)
   int cus = 256;                                  // compute units of the device
@@ -437,11 +453,23 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int choose_sub(int64_t n) { return n > 1024 ? 4 : 2; }
 
-// Tile size of one product: 256 x 256 for large n where the kernel exists,
-// else 128 (n > 1024) or 64.
-static int launch_sub(int64_t n, int fmt, int avg) {
+// Largest tile a product of size n may use (the plan pads n to a multiple of
+// it, so every smaller tile divides the padded size too): 256 x 256 from
+// n = 2048 where the kernel exists, else 128 (n > 1024) or 64.
+static int max_sub(int64_t n, int fmt, int avg) {
   if (n >= 2048 && sym_big_tile_capable(fmt, avg)) return 8;
   return choose_sub(n);
+}
+
+// Built-in tile choice (psdp_autotune can override it per size).  Measured
+// on MI355X for the 16-bit direct-to-LDS kernels: 256 x 256 tiles win from
+// n = 8192 (n = 8192: 32.4 ms per mixed projection against 32.5 ms), below
+// that too few of them fill the 256 CUs and 64 x 64 tiles win (n = 2048:
+// 1.66 ms against 4.35 ms with 256 x 256; n = 4096: 6.58 against 7.14 ms).
+// Other formats keep 128 x 128 above n = 1024.
+static int launch_sub(int64_t n, int fmt, int avg) {
+  if (!sym_big_tile_capable(fmt, avg)) return choose_sub(n);
+  return n >= 8192 ? 8 : 2;
 }
 
 static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
@@ -454,10 +482,10 @@ static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
   p.n = (int)n;
   p.sub = choose_sub(n);
   for (int i = 0; i < p.sch.nsteps; ++i) {
-    p.sub = std::max(p.sub, launch_sub(n, p.sch.fmt[i], 0));
-    p.sub = std::max(p.sub, launch_sub(n, p.sch.fmt[i], p.sch.avg[i]));
+    p.sub = std::max(p.sub, max_sub(n, p.sch.fmt[i], 0));
+    p.sub = std::max(p.sub, max_sub(n, p.sch.fmt[i], p.sch.avg[i]));
   }
-  p.sub = std::max(p.sub, launch_sub(n, p.sch.final_fmt, p.sch.final_avg));
+  p.sub = std::max(p.sub, max_sub(n, p.sch.final_fmt, p.sch.final_avg));
   p.bt = 32 * p.sub;  // the largest tile used: npad is a multiple of every tile
   p.npad = (int)((n + p.bt - 1) / p.bt * p.bt);
   p.ld = p.npad + LD_PAD;
@@ -535,6 +563,32 @@ static ProdDesc base_desc(const Plan& p) {
 }
 
 // Enqueues the whole projection.
+// One symmetric product on tiles of 32 * sub: sub 8 runs whole rounds of
+// 256 x 256 tiles and re-cuts a partial last round into 128 x 128 tiles.
+static psdp_status_t launch_product(psdp_handle_t h, const Plan& p, int fmt, int avg, const ProdDesc& d, int sub,
+                                    hipStream_t s) {
+  if (sub == 8) {
+    TileList big, small;
+    STCHK(get_split_tiles(h, p.npad / 256, big, small));
+    HIPCHK(launch_sym_product(fmt, 8, avg, d, big.dev, big.count, s));
+    if (small.count) HIPCHK(launch_sym_product(fmt, 4, avg, d, small.dev, small.count, s));
+    return PSDP_OK;
+  }
+  TileList tl;
+  STCHK(get_tiles(h, p.npad / (32 * sub), 32 * sub, tl));
+  HIPCHK(launch_sym_product(fmt, sub, avg, d, tl.dev, tl.count, s));
+  return PSDP_OK;
+}
+
+// Tile size of a product: the autotuner's choice for (n, format, averaged)
+// when psdp_autotune measured one, else the built-in rule.  Choices never
+// exceed the built-in tile, so the plan's padding holds for all of them.
+static int tuned_sub(psdp_handle_t h, const Plan& p, int fmt, int avg) {
+  auto it = h->tuned.find(std::make_tuple(p.n, fmt, avg));
+  if (it != h->tuned.end() && it->second <= max_sub(p.n, fmt, avg)) return it->second;
+  return launch_sub(p.n, fmt, avg);
+}
+
 // fixed_scale: s of the entry points without the bound (1 for the reference's
 // composite_* functions, the deflated matrix's scale for the *_deflate ones).
 static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
@@ -544,19 +598,8 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   // one product: tile size by format, then the launch
   auto product = [&](int fmt, int avg, const ProdDesc& d) -> psdp_status_t {
     DESCCHK(d, fmt);
-    const int sub = launch_sub(p.n, fmt, avg);
     ProfScope ps(h, PSDP_PROF_PROD_F32 + fmt);
-    if (sub == 8) {
-      TileList big, small;
-      STCHK(get_split_tiles(h, p.npad / 256, big, small));
-      HIPCHK(launch_sym_product(fmt, 8, avg, d, big.dev, big.count, s));
-      if (small.count) HIPCHK(launch_sym_product(fmt, 4, avg, d, small.dev, small.count, s));
-      return PSDP_OK;
-    }
-    TileList tl;
-    STCHK(get_tiles(h, p.npad / (32 * sub), 32 * sub, tl));
-    HIPCHK(launch_sym_product(fmt, sub, avg, d, tl.dev, tl.count, s));
-    return PSDP_OK;
+    return launch_product(h, p, fmt, avg, d, tuned_sub(h, p, fmt, avg), s);
   };
 
   {
@@ -778,6 +821,11 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   }
   if (h->arena) (void)hipFree(h->arena);
   if (h->keep) (void)hipFree(h->keep);
+  if (h->bgraph) (void)hipGraphExecDestroy(h->bgraph);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->aux) (void)hipStreamDestroy(h->aux);
   if (h->pinned) (void)hipHostFree(h->pinned);
   if (h->st) (void)hipFree(h->st);
   eig_destroy_blas(h->blas);
@@ -810,9 +858,111 @@ psdp_status_t psdp_set_seed(psdp_handle_t h, uint64_t seed) {
   return PSDP_OK;
 }
 
+psdp_status_t psdp_set_batched_graph(psdp_handle_t h, int on) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->batch_graph = on != 0;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_set_batched_tiling(psdp_handle_t h, int big_block_n, int concurrent) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (big_block_n < 0) return fail(PSDP_ERR_INVALID_ARG, "negative block size");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->batch_big_n = big_block_n;
+  h->batch_concurrent = concurrent != 0;
+  return PSDP_OK;
+}
+
 psdp_status_t psdp_synchronize(psdp_handle_t h) {
   if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
+  return PSDP_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+psdp_status_t psdp_autotune(psdp_handle_t h, int64_t n, const psdp_config_t* cfg) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  Plan p;
+  STCHK(make_plan(cfg, n, p));
+  void* base = nullptr;
+  STCHK(arena_reserve(h, p.total, &base));
+  Bufs b = carve(p, (char*)base);
+  hipStream_t s = h->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } guard{e0, e1};
+  HIPCHK(launch_fill_sym(b.a32, p.ld, p.n, p.npad, h->seed, s));
+  HIPCHK(launch_set_scale(h->st, 1.0, s));
+  // every (format, averaged) pair the schedule uses
+  std::vector<std::pair<int, int>> kinds;
+  for (int i = 0; i < p.sch.nsteps; ++i) {
+    kinds.emplace_back(p.sch.fmt[i], 0);
+    kinds.emplace_back(p.sch.fmt[i], p.sch.avg[i]);
+  }
+  kinds.emplace_back(p.sch.final_fmt, p.sch.final_avg);
+  std::sort(kinds.begin(), kinds.end());
+  kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
+  for (auto [fmt, avg] : kinds) {
+    HIPCHK(launch_scale_split(b.a32, p.N, &h->st->inv_scale, fmt == FMT_F32 ? FMT_F32 : fmt, b.ap,
+                              p.sch.final_fmt, b.x32, b.xp, s));
+    ProdDesc d = base_desc(p);
+    set_operand(d.x, fmt, b.a32, b.ap);
+    set_operand(d.y, fmt, b.a32, b.ap);
+    d.mode = EPI_STORE;
+    d.c32 = fmt == FMT_F32 ? b.p3 : nullptr;
+    d.out_fmt = fmt;
+    for (int q = 0; q < 3; ++q) d.outp[q] = b.p2p[q];
+    DESCCHK(d, fmt);
+    const int top = max_sub(p.n, fmt, avg);
+    int best = launch_sub(p.n, fmt, avg);
+    float best_ms = 1e30f;
+    for (int sub = top; sub >= 2; sub /= 2) {
+      STCHK(launch_product(h, p, fmt, avg, d, sub, s));  // warm-up (tile lists, caches)
+      constexpr int R = 3;
+      HIPCHK(hipEventRecord(e0, s));
+      for (int r = 0; r < R; ++r) STCHK(launch_product(h, p, fmt, avg, d, sub, s));
+      HIPCHK(hipEventRecord(e1, s));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) best_ms = ms, best = sub;
+    }
+    h->tuned[std::make_tuple(p.n, fmt, avg)] = best;
+  }
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_set_tuned_tile(psdp_handle_t h, int64_t n, int format, int average, int tile) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (tile != 0 && tile != 64 && tile != 128 && tile != 256)
+    return fail(PSDP_ERR_INVALID_ARG, "tile must be 64, 128, 256 (or 0 to clear)");
+  std::lock_guard<std::mutex> lock(h->mu);
+  const auto key = std::make_tuple((int)n, format, average ? 1 : 0);
+  if (tile == 0) h->tuned.erase(key);
+  else h->tuned[key] = tile / 32;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_tuned_tile(psdp_handle_t h, int64_t n, int format, int average, int* tile) {
+  if (!h || !tile) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(h->mu);
+  auto it = h->tuned.find(std::make_tuple((int)n, format, average));
+  *tile = it == h->tuned.end() ? 0 : 32 * it->second;
   return PSDP_OK;
 }
 
@@ -928,8 +1078,14 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
     return fail(PSDP_ERR_UNSUPPORTED, "averaged products exist for binary16 and bfloat16 operands only");
   std::lock_guard<std::mutex> lock(h->mu);
   HIPCHK(hipSetDevice(h->device));
-  const int sub = launch_sub(n, format, average ? 1 : 0), bt = 32 * sub;
-  const int npad = (int)((n + bt - 1) / bt * bt), ld = npad + LD_PAD;
+  // the projections' tile choice for this size (tuned or built in); n is
+  // padded for the largest tile, as in a projection plan
+  const int avg = average ? 1 : 0;
+  const int top = max_sub(n, format, avg);
+  auto it = h->tuned.find(std::make_tuple((int)n, format, avg));
+  const int sub = it != h->tuned.end() && it->second <= top ? it->second : launch_sub(n, format, avg);
+  const int bt = 32 * sub, btop = 32 * top;
+  const int npad = (int)((n + btop - 1) / btop * btop), ld = npad + LD_PAD;
   const int64_t N = (int64_t)npad * ld;
   const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
   void* base = nullptr;
@@ -989,22 +1145,62 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
 // batch 9% slower (short K loops, two launches in series), hence 2048.
 static constexpr int BIG_BLOCK_N = 2048;
 
+// Tile list of one class of a grouped launch.  Workgroups are dispatched
+// round-robin over the 8 XCDs and the persistent grid (a multiple of 8, or
+// one workgroup per entry) keeps that residue, so list position p runs on XCD
+// p % 8.  Blocks are spread over the XCDs by cost (T(T+1)/2 tiles of K =
+// npad, largest first onto the least loaded XCD) so that all tiles of a
+// block share one XCD's L2; an XCD's tiles are grouped by block with K
+// decreasing (the workgroups' static round-robin over them stays balanced);
+// the 8 lists are interleaved and the short ones padded with empty entries
+// (mat = -1).
+static std::vector<TileRef> batched_tile_order(const std::vector<int>& blocks, int bt,
+                                               const std::vector<int>& npad) {
+  auto npad_of = [&](int b) { return npad[b]; };
+  constexpr int XCD = 8;
+  std::vector<int> order(blocks);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return npad_of(a) > npad_of(b); });
+  std::vector<std::vector<TileRef>> lists(XCD);
+  double load[XCD] = {};
+  for (int b : order) {
+    const int T = npad_of(b) / bt;
+    const double cost = 0.5 * T * (T + 1) * (double)npad_of(b);
+    int x = 0;
+    for (int k = 1; k < XCD; ++k)
+      if (load[k] < load[x]) x = k;
+    load[x] += cost;
+    for (int i = 0; i < T; ++i)
+      for (int j = i; j < T; ++j) lists[x].push_back(TileRef{b, i, j, 0});
+  }
+  size_t len = 0;
+  for (auto& l : lists) len = std::max(len, l.size());
+  std::vector<TileRef> out;
+  out.reserve(len * XCD);
+  for (size_t k = 0; k < len; ++k)
+    for (int x = 0; x < XCD; ++x) out.push_back(k < lists[x].size() ? lists[x][k] : TileRef{-1, 0, 0, 0});
+  // trailing padding entries are not needed
+  while (!out.empty() && out.back().mat < 0) out.pop_back();
+  return out;
+}
+
 struct BatchPlan {
   Schedule sch;
   int iters = 20;
   double tol = 1e-10, margin = 1.0;
   int count = 0, nstages = 0, max_npad = 0;
   int ntiles_big = 0, ntiles_small = 0;  // 128 x 128 tiles first, then 64 x 64
+  int npad_tiles = 0, ngemv_rows = 0, nsplit_chunks = 0;  // flat work lists of the prep kernels
   int64_t max_N = 0;
   size_t off_states = 0, off_desc = 0, total = 0;
   size_t d_states = 0, d_lz = 0, d_pad = 0, d_split = 0, d_prod = 0, d_tiles = 0;
+  size_t d_padt = 0, d_rows = 0, d_chunks = 0;
   std::vector<char> blob;  // initial states + descriptors, built for one workspace base
 };
 
 // Lays out the workspace and, when base is given, builds every descriptor for
 // that base and checks them.  Pure host code: no device call.
 static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const psdp_config_t* cfg, char* base,
-                                  BatchPlan& bp) {
+                                  BatchPlan& bp, int big_n = BIG_BLOCK_N) {
   if (!blocks) return fail(PSDP_ERR_INVALID_ARG, "null argument");
   if (count <= 0 || count > 65535) return fail(PSDP_ERR_INVALID_ARG, "block count must be in [1, 65535]");
   Schedule& sch = bp.sch;
@@ -1039,7 +1235,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     q.p2 = q.x32 = kNone;
     for (int i = 0; i < 3; ++i) q.ap[i] = q.p2p[i] = q.p3p[i] = q.xp[i] = kNone;
     q.n = (int)blocks[b].n;
-    const int bt = q.n >= BIG_BLOCK_N ? 128 : 64;
+    const int bt = q.n >= big_n ? 128 : 64;
     q.npad = (q.n + bt - 1) / bt * bt;
     q.iters = std::min(bp.iters, q.n);
     bp.max_npad = std::max(bp.max_npad, q.npad);
@@ -1057,21 +1253,31 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     q.vec = off;
     off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
   }
-  // tiles of each class, the longest K first so that the workgroups' static
-  // round-robin over the list stays balanced
-  std::vector<TileRef> tiles, small;
-  for (int b = 0; b < count; ++b) {
-    const bool big = bb[b].n >= BIG_BLOCK_N;
-    const int T = bb[b].npad / (big ? 128 : 64);
-    for (int i = 0; i < T; ++i)
-      for (int j = i; j < T; ++j) (big ? tiles : small).push_back(TileRef{b, i, j, 0});
-  }
-  auto by_k = [&](const TileRef& x, const TileRef& y) { return bb[x.mat].npad > bb[y.mat].npad; };
-  std::stable_sort(tiles.begin(), tiles.end(), by_k);
-  std::stable_sort(small.begin(), small.end(), by_k);
+  // tiles of each class in XCD-aware order (batched_tile_order)
+  std::vector<int> big_blocks, small_blocks;
+  for (int b = 0; b < count; ++b) (bb[b].n >= big_n ? big_blocks : small_blocks).push_back(b);
+  std::vector<int> npads(count);
+  for (int b = 0; b < count; ++b) npads[b] = bb[b].npad;
+  std::vector<TileRef> tiles = batched_tile_order(big_blocks, 128, npads);
+  std::vector<TileRef> small = batched_tile_order(small_blocks, 64, npads);
   bp.ntiles_big = (int)tiles.size();
   bp.ntiles_small = (int)small.size();
   tiles.insert(tiles.end(), small.begin(), small.end());
+  // flat work lists: 64 x 64 tile pairs of the symmetrize/pad, row groups of
+  // the Lanczos GEMV, value chunks of the scale/split
+  std::vector<TileRef> padt;
+  std::vector<int2> rows, chunks;
+  for (int b = 0; b < count; ++b) {
+    const int T = bb[b].npad / 64;
+    for (int i = 0; i < T; ++i)
+      for (int j = i; j < T; ++j) padt.push_back(TileRef{b, i, j, 0});
+    for (int r = 0; r < bb[b].npad / GEMV_BATCH_ROWS; ++r) rows.push_back(make_int2(b, r));
+    const int64_t nc = (int64_t)bb[b].npad * bb[b].npad / SPLIT_CHUNK;
+    for (int64_t c = 0; c < nc; ++c) chunks.push_back(make_int2(b, (int)c));
+  }
+  bp.npad_tiles = (int)padt.size();
+  bp.ngemv_rows = (int)rows.size();
+  bp.nsplit_chunks = (int)chunks.size();
   bp.nstages = 3 * sch.nsteps + 1;
   bp.off_states = off;
   off += align256(sizeof(LanczosState) * count);
@@ -1083,6 +1289,9 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   bp.d_split = doff, doff += align256(sizeof(SplitJob) * count);
   bp.d_prod = doff, doff += align256(sizeof(ProdDesc) * (size_t)bp.nstages * count);
   bp.d_tiles = doff, doff += align256(sizeof(TileRef) * tiles.size());
+  bp.d_padt = doff, doff += align256(sizeof(TileRef) * padt.size());
+  bp.d_rows = doff, doff += align256(sizeof(int2) * rows.size());
+  bp.d_chunks = doff, doff += align256(sizeof(int2) * chunks.size());
   bp.total = off + doff;
   if (!base) return PSDP_OK;
 
@@ -1096,6 +1305,9 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   SplitJob* sj = (SplitJob*)(blob + bp.d_split);
   ProdDesc* pd = (ProdDesc*)(blob + bp.d_prod);
   memcpy(blob + bp.d_tiles, tiles.data(), sizeof(TileRef) * tiles.size());
+  memcpy(blob + bp.d_padt, padt.data(), sizeof(TileRef) * padt.size());
+  memcpy(blob + bp.d_rows, rows.data(), sizeof(int2) * rows.size());
+  memcpy(blob + bp.d_chunks, chunks.data(), sizeof(int2) * chunks.size());
 
   auto F = [&](size_t o) { return o != kNone ? (float*)(base + o) : (float*)nullptr; };
   auto U = [&](size_t o) { return o != kNone ? (uint16_t*)(base + o) : (uint16_t*)nullptr; };
@@ -1225,15 +1437,22 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
     psdp_default_config(&def, PSDP_METHOD_MIXED);
     cfg = &def;
   }
-  BatchPlan bp;
-  STCHK(plan_batched(blocks, count, cfg, nullptr, bp));
   std::lock_guard<std::mutex> lock(h->mu);
+  const int big_n = h->batch_big_n > 0 ? h->batch_big_n : BIG_BLOCK_N;
+  BatchPlan bp;
+  STCHK(plan_batched(blocks, count, cfg, nullptr, bp, big_n));
   HIPCHK(hipSetDevice(h->device));
   hipStream_t s = h->stream;
   void* base_v = nullptr;
   STCHK(arena_reserve(h, bp.total, &base_v));
   char* base = (char*)base_v;
-  STCHK(plan_batched(blocks, count, cfg, base, bp));
+  STCHK(plan_batched(blocks, count, cfg, base, bp, big_n));
+  const bool two = h->batch_concurrent && bp.ntiles_big && bp.ntiles_small;
+  if (two && !h->aux) {
+    HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
+  }
   const Schedule& sch = bp.sch;
   LanczosState* states = (LanczosState*)(base + bp.off_states);
   char* ddev = base + bp.off_desc;
@@ -1244,36 +1463,97 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   STCHK(staging_slot(h, bp.blob.size(), &stage));
   memcpy(stage, bp.blob.data(), bp.blob.size());
   HIPCHK(hipMemcpyAsync(ddev, stage, bp.blob.size(), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(states, ddev + bp.d_states, sizeof(LanczosState) * count, hipMemcpyDeviceToDevice, s));
   STCHK(staging_release(h));
 
   const LanczosJob* lz_d = (const LanczosJob*)(ddev + bp.d_lz);
   const ProdDesc* pd_d = (const ProdDesc*)(ddev + bp.d_prod);
   const TileRef* tl_d = (const TileRef*)(ddev + bp.d_tiles);
-  {
-    ProfScope ps(h, PSDP_PROF_PREP);
-    HIPCHK(launch_sym_pad_batched((const PadJob*)(ddev + bp.d_pad), count, bp.max_npad, s));
-  }
-  if (cfg->auto_scale) {
-    ProfScope ps(h, PSDP_PROF_LANCZOS);
-    HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin, s));
-  }
-  {
-    ProfScope ps(h, PSDP_PROF_PREP);
-    HIPCHK(launch_scale_split_batched((const SplitJob*)(ddev + bp.d_split), count, bp.max_N, sch.fmt[0],
-                                      sch.final_fmt, s));
-  }
-  for (int st = 0; st < bp.nstages; ++st) {
-    const bool fin = st >= 3 * sch.nsteps;
-    const int f = fin ? sch.final_fmt : sch.fmt[st / 3];
-    const int avg = fin ? sch.final_avg : (st % 3 != 0 ? sch.avg[st / 3] : 0);
-    ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
-    const ProdDesc* pd = pd_d + (size_t)st * count;
-    if (bp.ntiles_big)
-      HIPCHK(launch_sym_product_grouped(f, 4, avg, pd, tl_d, bp.ntiles_big, std::min(bp.ntiles_big, h->cus), s));
-    if (bp.ntiles_small)
-      HIPCHK(launch_sym_product_grouped(f, 2, avg, pd, tl_d + bp.ntiles_big, bp.ntiles_small,
-                                        std::min(bp.ntiles_small, 4 * h->cus), s));
+  static_assert(sizeof(LanczosState) % 8 == 0, "states are copied in 8-byte words");
+  // everything after the upload reads only device memory: one fixed sequence
+  // of launches, which graph mode captures once and replays
+  auto enqueue = [&](hipStream_t s) -> psdp_status_t {
+    HIPCHK(launch_copy_words(states, ddev + bp.d_states, (int64_t)(sizeof(LanczosState) * count / 8), s));
+    {
+      ProfScope ps(h, PSDP_PROF_PREP);
+      HIPCHK(launch_sym_pad_batched((const PadJob*)(ddev + bp.d_pad), (const TileRef*)(ddev + bp.d_padt),
+                                    bp.npad_tiles, s));
+    }
+    if (cfg->auto_scale) {
+      ProfScope ps(h, PSDP_PROF_LANCZOS);
+      HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin,
+                                    (const int2*)(ddev + bp.d_rows), bp.ngemv_rows, s));
+    }
+    {
+      ProfScope ps(h, PSDP_PROF_PREP);
+      HIPCHK(launch_scale_split_batched((const SplitJob*)(ddev + bp.d_split), (const int2*)(ddev + bp.d_chunks),
+                                        bp.nsplit_chunks, sch.fmt[0], sch.final_fmt, s));
+    }
+    for (int st = 0; st < bp.nstages; ++st) {
+      const bool fin = st >= 3 * sch.nsteps;
+      const int f = fin ? sch.final_fmt : sch.fmt[st / 3];
+      const int avg = fin ? sch.final_avg : (st % 3 != 0 ? sch.avg[st / 3] : 0);
+      ProfScope ps(h, PSDP_PROF_PROD_F32 + f);
+      const ProdDesc* pd = pd_d + (size_t)st * count;
+      // the small-tile class runs beside the big one on the second stream
+      hipStream_t s2 = two ? h->aux : s;
+      if (two) {
+        HIPCHK(hipEventRecord(h->fork_ev, s));
+        HIPCHK(hipStreamWaitEvent(h->aux, h->fork_ev, 0));
+      }
+      if (bp.ntiles_big)
+        HIPCHK(launch_sym_product_grouped(f, 4, avg, pd, tl_d, bp.ntiles_big, std::min(bp.ntiles_big, h->cus), s));
+      if (bp.ntiles_small)
+        HIPCHK(launch_sym_product_grouped(f, 2, avg, pd, tl_d + bp.ntiles_big, bp.ntiles_small,
+                                          std::min(bp.ntiles_small, 4 * h->cus), s2));
+      if (two) {
+        HIPCHK(hipEventRecord(h->join_ev, h->aux));
+        HIPCHK(hipStreamWaitEvent(s, h->join_ev, 0));
+      }
+    }
+    return PSDP_OK;
+  };
+  if (!h->batch_graph) {
+    STCHK(enqueue(s));
+  } else {
+    // the graph holds kernel arguments (workspace addresses, sizes, seed,
+    // streams): replay it only for the same plan on the same workspace
+    auto bits = [](double v) {
+      int64_t b;
+      memcpy(&b, &v, sizeof b);
+      return b;
+    };
+    std::vector<int64_t> key = {count, big_n, two, (int64_t)(intptr_t)base, (int64_t)h->seed,
+                                (int64_t)(intptr_t)h->aux, cfg->auto_scale, bp.iters,
+                                bits(bp.tol), bits(bp.margin), sch.nsteps, sch.final_fmt, sch.final_avg};
+    for (int i = 0; i < sch.nsteps; ++i)
+      for (int64_t v : {(int64_t)sch.fmt[i], (int64_t)sch.avg[i], bits(sch.a[i]), bits(sch.b[i]), bits(sch.c[i])})
+        key.push_back(v);
+    for (int b = 0; b < count; ++b) key.push_back(blocks[b].n);
+    if (!h->bgraph || key != h->bgraph_key) {
+      if (h->bgraph) {
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGraphExecDestroy(h->bgraph));
+        h->bgraph = nullptr;
+      }
+      // captured on a stream of the handle's own (the caller's stream may be
+      // the legacy default stream, which cannot capture); replayed on s
+      if (!h->cap) HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+      const bool prof = h->profiling;
+      h->profiling = false;  // no timing events inside the graph
+      HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+      const psdp_status_t est = enqueue(h->cap);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(h->cap, &g);
+      h->profiling = prof;
+      STCHK(est);
+      HIPCHK(ce);
+      const hipError_t ie = hipGraphInstantiate(&h->bgraph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(ie);
+      h->bgraph_key = key;
+    }
+    ProfScope ps(h, PSDP_PROF_GRAPH);
+    HIPCHK(hipGraphLaunch(h->bgraph, s));
   }
   if (cfg->synchronize) {
     std::vector<LanczosState> out(count);
@@ -1283,6 +1563,44 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
       if (out[b].nonfinite) return fail(PSDP_ERR_NONFINITE, "block %d: non-finite value in the input", b);
   }
   return PSDP_OK;
+}
+
+psdp_status_t psdp_autotune_batched(psdp_handle_t h, const psdp_block_t* blocks, int count,
+                                    const psdp_config_t* cfg) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  psdp_config_t c;
+  if (cfg) c = *cfg;
+  else psdp_default_config(&c, PSDP_METHOD_MIXED);
+  c.synchronize = 0;
+  // candidate tile classes: the 128 x 128 class from n >= big (or none), the
+  // two classes of a stage serial or on two streams
+  const int cand[][2] = {{BIG_BLOCK_N, 0}, {1024, 0}, {768, 0}, {512, 0}, {512, 1}, {384, 1}, {256, 1}};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } guard{e0, e1};
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int k = 0; k < (int)(sizeof cand / sizeof cand[0]); ++k) {
+    STCHK(psdp_set_batched_tiling(h, cand[k][0], cand[k][1]));
+    STCHK(psdp_project_batched_f32(h, blocks, count, &c));  // warm-up: workspace, tile lists
+    constexpr int R = 2;
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int r = 0; r < R; ++r) STCHK(psdp_project_batched_f32(h, blocks, count, &c));
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) best_ms = ms, best = k;
+  }
+  return psdp_set_batched_tiling(h, cand[best][0], cand[best][1]);
 }
 
 static psdp_status_t ensure_blas(psdp_handle_t h) {

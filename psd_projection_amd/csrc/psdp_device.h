@@ -99,6 +99,17 @@ __device__ __forceinline__ void store_operand(int fmt, float v, int64_t idx, flo
 }
 
 // ---------------------------------------------------------------------------
+// Global-memory view of a pointer that was read from device memory (a field
+// of a launch descriptor).  Without it the compiler cannot prove the address
+// space and emits FLAT accesses, which also count on the LDS counter: every
+// wait for an LDS read would then wait for all outstanding stores as well.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// ---------------------------------------------------------------------------
 // Seeded counter-based uniform stream: element i of stream `seed` is
 // (splitmix64(seed ^ C (i + 1)) >> 11 + 1) 2^-53, a value in (0, 1].
 // ---------------------------------------------------------------------------

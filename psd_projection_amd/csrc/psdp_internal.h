@@ -61,6 +61,11 @@ struct TileRef {
 
 constexpr int LANCZOS_MAX_ITER = 64;
 
+// Rows per workgroup of the batched GEMV, values per workgroup of the batched
+// scale/split (a 64 x 64 tile holds a whole number of chunks).
+constexpr int GEMV_BATCH_ROWS = 16;
+constexpr int SPLIT_CHUNK = 4096;
+
 // Device-resident Lanczos state (one per matrix).
 struct LanczosState {
   double alpha[LANCZOS_MAX_ITER];

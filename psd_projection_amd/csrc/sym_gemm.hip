@@ -120,10 +120,10 @@ __device__ __forceinline__ void store_pieces4(int fmt, uint16_t* const* outp, in
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (q < np) {
-      uint2 pk;
-      pk.x = (uint32_t)p[0][q] | ((uint32_t)p[1][q] << 16);
-      pk.y = (uint32_t)p[2][q] | ((uint32_t)p[3][q] << 16);
-      *(uint2*)(outp[q] + idx) = pk;
+      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+      const u32x2 pk = {(uint32_t)p[0][q] | ((uint32_t)p[1][q] << 16),
+                        (uint32_t)p[2][q] | ((uint32_t)p[3][q] << 16)};
+      *(__attribute__((address_space(1))) u32x2*)gbl(outp[q] + idx) = pk;
     }
   }
 }
@@ -132,9 +132,9 @@ __device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int
   uint16_t p[3];
   split_value(fmt, w, p);
   const int np = fmt_pieces(fmt);
-  outp[0][idx] = p[0];
-  if (np > 1) outp[1][idx] = p[1];
-  if (np > 2) outp[2][idx] = p[2];
+  gbl(outp[0])[idx] = p[0];
+  if (np > 1) gbl(outp[1])[idx] = p[1];
+  if (np > 2) gbl(outp[2])[idx] = p[2];
 }
 
 // Epilogue for tile rows [r0, r0 + NR) staged in LDS (C holds local rows
@@ -145,6 +145,10 @@ __device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int
 // the diagonal), so the phases of one tile do not interfere.
 // cscale multiplies the staged values (the 2^-16 of FP16x3 products when the
 // tile stored raw accumulators).
+// Rows one epilogue_rows call covers at most (the LDS C tile holds 128 rows).
+template <int BT>
+constexpr int NR_MAX = BT < 128 ? BT : 128;
+
 template <int BT, int NT = 256>
 __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
                                               float cscale = 1.0f) {
@@ -155,7 +159,11 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const bool diag = bi == bj;
   const int64_t ld = d.ld;
   const int nq = NR / 4;
+  auto c32 = gbl(d.c32);
+  auto p3 = gbl(d.p3);
   if (d.mode == EPI_OUTPUT) {
+    auto out64 = gbl((double*)d.out);
+    auto out32 = gbl((float*)d.out);
     const double s = (d.scale ? *d.scale : 1.0) * (double)cscale;
     const int64_t old = d.out_ld;
     // (i, j) at i + j*ld: contiguous along i
@@ -167,8 +175,8 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
         const int i = row0 + rr + e;
         if (i < d.n && j < d.n && (!diag || i <= j)) {
           const double v = (double)C[(rr + e) * CS + c] * s;
-          if (d.out_f64) ((double*)d.out)[(int64_t)i + (int64_t)j * old] = v;
-          else ((float*)d.out)[(int64_t)i + (int64_t)j * old] = (float)v;
+          if (d.out_f64) out64[(int64_t)i + (int64_t)j * old] = v;
+          else out32[(int64_t)i + (int64_t)j * old] = (float)v;
         }
       }
     }
@@ -181,8 +189,8 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
         const int j = col0 + c0 + e;
         if (i < d.n && j < d.n && (!diag || i < j)) {
           const double v = (double)C[r * CS + c0 + e] * s;
-          if (d.out_f64) ((double*)d.out)[(int64_t)j + (int64_t)i * old] = v;
-          else ((float*)d.out)[(int64_t)j + (int64_t)i * old] = (float)v;
+          if (d.out_f64) out64[(int64_t)j + (int64_t)i * old] = v;
+          else out32[(int64_t)j + (int64_t)i * old] = (float)v;
         }
       }
     }
@@ -191,8 +199,27 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const bool poly = d.mode == EPI_POLY || d.mode == EPI_POLY_HALF_ID;
   const bool half_id = d.mode == EPI_POLY_HALF_ID;
   const bool pieces = d.out_fmt != FMT_F32;
-  // pass 1: primary block, rows of the tile
-  for (int q = tid; q < NR * Q4; q += NT) {
+  // pass 1: primary block, rows of the tile.  The polynomial's operands
+  // (A and A^3 of this block) are loaded G quads ahead, so that a thread has
+  // G loads in flight instead of waiting out each one before its stores.
+  constexpr int G = (NR_MAX<BT> * Q4 / NT) < 8 ? (NR_MAX<BT> * Q4 / NT > 0 ? NR_MAX<BT> * Q4 / NT : 1) : 8;
+  for (int q0 = tid; q0 < NR * Q4; q0 += NT * G) {
+    f32x4 a4g[G], p4g[G];
+    if (poly) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int q = q0 + g * NT;
+        if (q < NR * Q4) {
+          const int64_t idx = (int64_t)(row0 + q / Q4) * ld + col0 + (q % Q4) * 4;
+          a4g[g] = *(const __attribute__((address_space(1))) f32x4*)(c32 + idx);
+          p4g[g] = *(const __attribute__((address_space(1))) f32x4*)(p3 + idx);
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+    const int q = q0 + g * NT;
+    if (q >= NR * Q4) break;
     const int r = q / Q4, c0 = (q % Q4) * 4;
     const int i = row0 + r, j0 = col0 + c0;
     const int64_t idx = (int64_t)i * ld + j0;
@@ -200,8 +227,8 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
 #pragma unroll
     for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
     if (poly) {
-      const f32x4 a4 = *(const f32x4*)(d.c32 + idx);
-      const f32x4 p4 = *(const f32x4*)(d.p3 + idx);
+      const f32x4 a4 = a4g[g];
+      const f32x4 p4 = p4g[g];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         float x = d.a * a4[e];
@@ -212,13 +239,13 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
       }
     }
     if (!diag) {
-      if (d.c32) *(f32x4*)(d.c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
+      if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
       if (pieces) store_pieces4(d.out_fmt, d.outp, idx, w);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (i <= j0 + e) {
-          if (d.c32) d.c32[idx + e] = w[e];
+          if (c32) c32[idx + e] = w[e];
           if (pieces) store_piece1(d.out_fmt, d.outp, idx + e, w[e]);
         }
       }
@@ -226,6 +253,7 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
     if (poly || cscale != 1.0f) {  // pass 2 reads final values
 #pragma unroll
       for (int e = 0; e < 4; ++e) C[r * CS + c0 + e] = w[e];
+    }
     }
   }
   __syncthreads();
@@ -238,13 +266,13 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
 #pragma unroll
     for (int e = 0; e < 4; ++e) w[e] = C[(rr + e) * CS + c];
     if (!diag) {
-      if (d.c32) *(f32x4*)(d.c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
+      if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
       if (pieces) store_pieces4(d.out_fmt, d.outp, idx, w);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (i0 + e < j) {
-          if (d.c32) d.c32[idx + e] = w[e];
+          if (c32) c32[idx + e] = w[e];
           if (pieces) store_piece1(d.out_fmt, d.outp, idx + e, w[e]);
         }
       }
@@ -713,6 +741,7 @@ __global__ __launch_bounds__(256) void sym_prod_dma_grouped_kernel(const ProdDes
   __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB>::LDS_BYTES];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
+    if (tr.mat < 0) continue;  // padding entry of an XCD-interleaved list
     sym_tile_dma<FMT, SUB, false>(descs[tr.mat], tr.bi, tr.bj, smem);  // descriptor read in place
   }
 }
@@ -737,6 +766,7 @@ __global__ __launch_bounds__(256) void sym_prod_grouped_kernel(const ProdDesc* _
   __shared__ __attribute__((aligned(16))) typename TL::E smem[TL::SMEM_ELEMS];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileRef tr = tiles[t];
+    if (tr.mat < 0) continue;  // padding entry of an XCD-interleaved list
     const ProdDesc d = descs[tr.mat];
     sym_tile<FMT, SUB, AVG>(d, tr.bi, tr.bj, smem);
   }

@@ -63,11 +63,18 @@ def test_sym_product_split_tail(handle, fmt):
     # n = 6144: 300 tiles of 256 x 256, i.e. one round on 256 CUs plus a partial
     # one that runs as 128 x 128 tiles in a second launch (on devices with
     # fewer CUs the split differs; the result must not).
+    # The built-in choice at this size is 64 x 64 tiles; 256 x 256 is set by
+    # hand, and both must give the same bits.
     n = 6144
     print(f"sym_product split tail fmt={fmt} n={n}", flush=True)
     X, _, _ = O.spectral_input(n, 77, "uniform")
     X32 = X.astype(np.float32)
-    C = from_dev(P.sym_product(handle, to_dev(X32, torch.float32), to_dev(X32, torch.float32), fmt))
+    Xd = to_dev(X32, torch.float32)
+    handle.set_tile(n, fmt, 256)
+    C = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+    handle.set_tile(n, fmt, 0)
+    C64 = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+    assert np.array_equal(C, C64)
     ref = O.sym_product(X32, X32, fmt, "mirror").astype(np.float64)
     assert np.array_equal(C, C.T)
     err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
