@@ -123,6 +123,7 @@ def bench_batched(args, P, W, torch):
     h = P.Handle(0, seed=4)
     cfg = P.default_config(args.method, synchronize=0)
     outs = P.project_batched(h, xs, config=cfg)
+    h.autotune_batched(xs, outs, config=cfg)  # tile classes (not timed)
     torch.cuda.synchronize()
     reps = 5
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -141,7 +142,66 @@ def bench_batched(args, P, W, torch):
     if flops:
         res["mfma_tflops"] = round(flops / (ms * 1e-3) / 1e12, 1)
         res["mfma_frac"] = round(flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["f16"], 4)
+        # HBM bytes a batch cannot avoid with one launch per product: per padded
+        # element, each product's epilogue writes its result (pieces; float32
+        # copies of A^3 and of the new A for the polynomial) and the polynomial
+        # reads A and A^3 in float32, and each product reads its two operands
+        # once; 40 Lanczos GEMVs read the float32 matrix.
+        npad2 = sum(float(((n + 63) // 64 * 64) ** 2) for n in sizes)
+        per_step = {"f16x3": 4 + (4 + 4) + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
+        steps = ["f16x3"] * 4 + ["f16"] * 3
+        bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + npad2 * 4 * 41
+        t_hbm = bytes_batch / 8.0e12 * 1e3
+        t_mfma = flops / (PEAK_TFLOPS["f16"] * 1e12) * 1e3
+        res["roofline"] = {"bound": "hbm" if t_hbm > t_mfma else "mfma", "model_ms_hbm": round(t_hbm, 3),
+                           "model_ms_mfma": round(t_mfma, 3),
+                           "frac_of_bound": round(max(t_hbm, t_mfma) / ms, 4),
+                           "model_bytes": bytes_batch,
+                           "note": "blocks n <= 1024: each product's epilogue traffic (~30 B per element) "
+                                   "outweighs its 2n flops per element, so the batch is HBM-bound"}
     log(f"batched: {res}")
+    return res
+
+
+def bench_deflate(args, P, W, torch):
+    """Deflated projections (compute_eigenpairs / lobpcg + chain + rank-k
+    updates) on the spectrum of the reference's own deflation tests
+    (generateAndProject(n, 10), test/test_utils.hpp:20-164: eigenvalues evenly
+    spaced in [-10, 10]) with a Haar basis, float64, k = 40."""
+    n = args.deflate
+    Ad, Pd, _ = W.spectral_matrix(n, 11, "even10")
+    h = P.Handle(0, seed=7)
+    res = {"workload": f"n={n}, eigenvalues evenly spaced in [-10, 10], Haar Q, float64, k = 40"}
+    legs = (("fp32_deflate", lambda d: P.composite_FP32_auto_scale_deflate(h, d, k=40, ortho_tol=1e-5)),
+            ("fp16_deflate", lambda d: P.composite_FP16_auto_scale_deflate(h, d, k=40)),
+            ("eig_fp64_deflate", lambda d: P.eig_FP64_deflate(h, d, k=40)),
+            ("fp32_auto_scale", lambda d: P.composite_FP32_auto_scale(h, d)),
+            ("eig_fp64", lambda d: P.eig_FP64_psd(h, d)))
+    for name, fn in legs:
+        d = Ad.clone()
+        fn(d)
+        torch.cuda.synchronize()
+        err = float((d - Pd).norm() / Pd.norm())
+        ts = []
+        for _ in range(2):
+            d = Ad.clone()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn(d)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        res[name] = {"ms": round(min(ts), 3), "rel_err": err}
+    P.lobpcg(h, Ad, 40)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    P.lobpcg(h, Ad, 40)
+    torch.cuda.synchronize()
+    lob_ms = (time.perf_counter() - t) * 1e3
+    it, rn = h.lobpcg_last()
+    res["lobpcg_m40"] = {"ms": round(lob_ms, 3), "iterations": it, "ms_per_iteration": round(lob_ms / max(it, 1), 3),
+                         "residual_norm": rn}
+    log(f"deflate: {res}")
+    h.close()
     return res
 
 
@@ -219,6 +279,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="config 5 requests in the timed Poisson stream (0: skip; the full config is 10000)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
+    ap.add_argument("--deflate", type=int, default=1024, help="size of the deflated-projection leg (0: skip)")
     ap.add_argument("--basis", default="haar", choices=("haar", "householder"),
                     help="eigenvector basis of the synthetic input (householder: no QR, for counter runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -261,10 +322,11 @@ def main():
     Y = torch.empty_like(X)
     cfg = P.default_config(args.method, synchronize=0)
 
+    tuned = h.autotune(n, config=cfg)  # tile size per operand format (not timed)
     for _ in range(args.warmup):
         P.project(h, X, out=Y, config=cfg)
     torch.cuda.synchronize()
-    log("warmup done")
+    log(f"warmup done; tiles chosen by the autotuner {tuned}")
 
     h.profile(True)
     start = torch.cuda.Event(enable_timing=True)
@@ -293,8 +355,9 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[fmt],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
                 "traffic": measured_traffic(n, fmt),
-                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel: 256x256 tiles in whole "
-                          "rounds + the last partial round as 128x128 tiles in a second launch)",
+                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel, autotuned tiles "
+                          f"{sorted(set(tuned.values()))}; 256x256 tiles run in whole rounds + the last partial "
+                          "round as 128x128 tiles in a second launch)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
                 "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
     total_chain_flops = sum(PRODUCTS[f] * wsym * c for f, (c, _) in prods.items()) / args.steps
@@ -322,6 +385,7 @@ def main():
 
     if args.fp32_steps > 0:
         cfg32 = P.default_config("composite_FP32", synchronize=0)
+        h.autotune(n, config=cfg32)
         P.project(h, X, out=Y, config=cfg32)
         torch.cuda.synchronize()
         start.record()
@@ -343,6 +407,8 @@ def main():
         out["batched"] = bench_batched(args, P, W, torch)
     if args.serve_requests > 0:
         out["served"] = bench_served(args, P, W, torch)
+    if args.deflate > 0:
+        out["deflate"] = bench_deflate(args, P, W, torch)
 
     if cpu_block is not None:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

@@ -96,11 +96,15 @@ class TorchExecutor:
         return self.lanes[lane][0]
 
     def warm(self, sizes):
-        """Sizes the lane arenas and tile lists for the largest requests
-        (keeps allocation out of the timed loop)."""
+        """Tunes the large lanes' tile sizes and sizes the lane arenas and tile
+        lists for the largest requests (keeps both out of the timed loop)."""
         torch = self.torch
         small = [n for n in sizes if n <= 1024]
         large = [n for n in sizes if n > 1024]
+        for lane in range(len(self.lanes)):
+            if not self.is_small_lane(lane):
+                for n in large:  # tile sizes of the single-matrix projections
+                    self.lanes[lane][1].autotune(n, config=self.cfg)
         for lane in range(len(self.lanes)):
             ns = small if self.is_small_lane(lane) else large
             if not ns:

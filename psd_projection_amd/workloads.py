@@ -14,7 +14,11 @@ def spectrum(n: int, seed: int, kind: str) -> np.ndarray:
     """Eigenvalues of the configurations:
     uniform: U[-1,1] (configs 1, 4); cfg2: N(0,1) scaled to max|lam| = 10 with
     30% replaced by U[-1e-2,1e-2]; cfg3: U[-5,5] with 10% replaced by
-    U[-1e-3,1e-3]; normal: N(0,1) (config 5)."""
+    U[-1e-3,1e-3]; normal: N(0,1) (config 5); even10: evenly spaced in
+    [-10, 10], the spectrum of the reference's test generator
+    (test/test_utils.hpp:20-164, radius 10)."""
+    if kind == "even10":
+        return -10.0 + 20.0 * np.arange(n) / float(n - 1)
     rng = np.random.default_rng(seed + 100)
     if kind == "uniform":
         return rng.uniform(-1.0, 1.0, n)
