@@ -123,7 +123,8 @@ def bench_batched(args, P, W, torch):
     h = P.Handle(0, seed=4)
     cfg = P.default_config(args.method, synchronize=0)
     outs = P.project_batched(h, xs, config=cfg)
-    h.autotune_batched(xs, outs, config=cfg)  # tile classes (not timed)
+    if args.autotune:
+        h.autotune_batched(xs, outs, config=cfg)  # tile classes (not timed)
     torch.cuda.synchronize()
     reps = 5
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -280,6 +281,7 @@ def main():
                     help="config 5 requests in the timed Poisson stream (0: skip; the full config is 10000)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
     ap.add_argument("--deflate", type=int, default=1024, help="size of the deflated-projection leg (0: skip)")
+    ap.add_argument("--autotune", type=int, default=1, help="tune tile sizes before timing (0: built-in tiles)")
     ap.add_argument("--basis", default="haar", choices=("haar", "householder"),
                     help="eigenvector basis of the synthetic input (householder: no QR, for counter runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -322,7 +324,7 @@ def main():
     Y = torch.empty_like(X)
     cfg = P.default_config(args.method, synchronize=0)
 
-    tuned = h.autotune(n, config=cfg)  # tile size per operand format (not timed)
+    tuned = h.autotune(n, config=cfg) if args.autotune else {}  # tile size per operand format (not timed)
     for _ in range(args.warmup):
         P.project(h, X, out=Y, config=cfg)
     torch.cuda.synchronize()
@@ -356,7 +358,7 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
                 "traffic": measured_traffic(n, fmt),
                 "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel, autotuned tiles "
-                          f"{sorted(set(tuned.values()))}; 256x256 tiles run in whole rounds + the last partial "
+                          f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds + the last partial "
                           "round as 128x128 tiles in a second launch)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
                 "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
@@ -385,7 +387,8 @@ def main():
 
     if args.fp32_steps > 0:
         cfg32 = P.default_config("composite_FP32", synchronize=0)
-        h.autotune(n, config=cfg32)
+        if args.autotune:
+            h.autotune(n, config=cfg32)
         P.project(h, X, out=Y, config=cfg32)
         torch.cuda.synchronize()
         start.record()
