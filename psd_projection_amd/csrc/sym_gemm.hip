@@ -112,31 +112,6 @@ __device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c, int fmt) {
                                                 __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
-__device__ __forceinline__ void store_pieces4(int fmt, uint16_t* const* outp, int64_t idx, const float w[4]) {
-  uint16_t p[4][3];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) split_value(fmt, w[e], p[e]);
-  const int np = fmt_pieces(fmt);
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    if (q < np) {
-      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-      const u32x2 pk = {(uint32_t)p[0][q] | ((uint32_t)p[1][q] << 16),
-                        (uint32_t)p[2][q] | ((uint32_t)p[3][q] << 16)};
-      *(__attribute__((address_space(1))) u32x2*)gbl(outp[q] + idx) = pk;
-    }
-  }
-}
-
-__device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int64_t idx, float w) {
-  uint16_t p[3];
-  split_value(fmt, w, p);
-  const int np = fmt_pieces(fmt);
-  gbl(outp[0])[idx] = p[0];
-  if (np > 1) gbl(outp[1])[idx] = p[1];
-  if (np > 2) gbl(outp[2])[idx] = p[2];
-}
-
 // Epilogue for tile rows [r0, r0 + NR) staged in LDS (C holds local rows
 // 0..NR-1, row stride BT + 1).  The primary block (i, j) is written along j,
 // the mirror block (j, i) along i, so both are coalesced 16-byte stores.
@@ -149,60 +124,47 @@ __device__ __forceinline__ void store_piece1(int fmt, uint16_t* const* outp, int
 template <int BT>
 constexpr int NR_MAX = BT < 128 ? BT : 128;
 
-template <int BT, int NT = 256>
-__device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
-                                              float cscale = 1.0f) {
+// Primary and mirror passes of a polynomial / store epilogue with the output
+// operand format OF fixed at compile time, so that the per-element split is
+// straight-line code and the loops stay compact (the instruction cache holds
+// them; a runtime format switch unrolled per element did not fit).
+template <int BT, int NT, int OF>
+__device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, float cscale, int row0, int col0,
+                                                    bool diag, int64_t ld,
+                                                    __attribute__((address_space(1))) float* c32,
+                                                    __attribute__((address_space(1))) const float* p3,
+                                                    uint16_t* const outp[3], bool poly, bool half_id, float ca,
+                                                    float cb, float cc) {
   constexpr int CS = BT + 1;
   constexpr int Q4 = BT / 4;
+  constexpr bool pieces = OF != FMT_F32;
+  constexpr int NP = OF == FMT_F16X3 || OF == FMT_BF16X3 ? 2 : (OF == FMT_BF16X9 ? 3 : 1);
   const int tid = threadIdx.x;
-  const int row0 = bi * BT + r0, col0 = bj * BT;
-  const bool diag = bi == bj;
-  const int64_t ld = d.ld;
   const int nq = NR / 4;
-  auto c32 = gbl(d.c32);
-  auto p3 = gbl(d.p3);
-  if (d.mode == EPI_OUTPUT) {
-    auto out64 = gbl((double*)d.out);
-    auto out32 = gbl((float*)d.out);
-    const double s = (d.scale ? *d.scale : 1.0) * (double)cscale;
-    const int64_t old = d.out_ld;
-    // (i, j) at i + j*ld: contiguous along i
-    for (int q = tid; q < BT * nq; q += NT) {
-      const int c = q / nq, rr = (q % nq) * 4;
-      const int j = col0 + c;
+  __attribute__((address_space(1))) uint16_t* op[3] = {gbl(outp[0]), gbl(outp[1]), gbl(outp[2])};
+  auto put4 = [&](int64_t idx, const float w[4]) {
+    uint16_t pc[4][3];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = row0 + rr + e;
-        if (i < d.n && j < d.n && (!diag || i <= j)) {
-          const double v = (double)C[(rr + e) * CS + c] * s;
-          if (d.out_f64) out64[(int64_t)i + (int64_t)j * old] = v;
-          else out32[(int64_t)i + (int64_t)j * old] = (float)v;
-        }
-      }
-    }
-    // (j, i) at j + i*ld: contiguous along j
-    for (int q = tid; q < NR * Q4; q += NT) {
-      const int r = q / Q4, c0 = (q % Q4) * 4;
-      const int i = row0 + r;
+    for (int e = 0; e < 4; ++e) split_value(OF, w[e], pc[e]);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int j = col0 + c0 + e;
-        if (i < d.n && j < d.n && (!diag || i < j)) {
-          const double v = (double)C[r * CS + c0 + e] * s;
-          if (d.out_f64) out64[(int64_t)j + (int64_t)i * old] = v;
-          else out32[(int64_t)j + (int64_t)i * old] = (float)v;
-        }
-      }
+    for (int q = 0; q < NP; ++q) {
+      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+      const u32x2 pk = {(uint32_t)pc[0][q] | ((uint32_t)pc[1][q] << 16),
+                        (uint32_t)pc[2][q] | ((uint32_t)pc[3][q] << 16)};
+      *(__attribute__((address_space(1))) u32x2*)(op[q] + idx) = pk;
     }
-    return;
-  }
-  const bool poly = d.mode == EPI_POLY || d.mode == EPI_POLY_HALF_ID;
-  const bool half_id = d.mode == EPI_POLY_HALF_ID;
-  const bool pieces = d.out_fmt != FMT_F32;
+  };
+  auto put1 = [&](int64_t idx, float w) {
+    uint16_t pc[3];
+    split_value(OF, w, pc);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) op[q][idx] = pc[q];
+  };
   // pass 1: primary block, rows of the tile.  The polynomial's operands
   // (A and A^3 of this block) are loaded G quads ahead, so that a thread has
   // G loads in flight instead of waiting out each one before its stores.
   constexpr int G = (NR_MAX<BT> * Q4 / NT) < 8 ? (NR_MAX<BT> * Q4 / NT > 0 ? NR_MAX<BT> * Q4 / NT : 1) : 8;
+#pragma unroll 1
   for (int q0 = tid; q0 < NR * Q4; q0 += NT * G) {
     f32x4 a4g[G], p4g[G];
     if (poly) {
@@ -218,46 +180,47 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-    const int q = q0 + g * NT;
-    if (q >= NR * Q4) break;
-    const int r = q / Q4, c0 = (q % Q4) * 4;
-    const int i = row0 + r, j0 = col0 + c0;
-    const int64_t idx = (int64_t)i * ld + j0;
-    float w[4];
+      const int q = q0 + g * NT;
+      if (q >= NR * Q4) break;
+      const int r = q / Q4, c0 = (q % Q4) * 4;
+      const int i = row0 + r, j0 = col0 + c0;
+      const int64_t idx = (int64_t)i * ld + j0;
+      float w[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
-    if (poly) {
-      const f32x4 a4 = a4g[g];
-      const f32x4 p4 = p4g[g];
+      for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
+      if (poly) {
+        const f32x4 a4 = a4g[g];
+        const f32x4 p4 = p4g[g];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        float x = d.a * a4[e];
-        x = fmaf(d.b, p4[e], x);
-        x = fmaf(d.c, w[e], x);
-        if (half_id) x = (x + (i == j0 + e ? 1.0f : 0.0f)) * 0.5f;
-        w[e] = x;
-      }
-    }
-    if (!diag) {
-      if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
-      if (pieces) store_pieces4(d.out_fmt, d.outp, idx, w);
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (i <= j0 + e) {
-          if (c32) c32[idx + e] = w[e];
-          if (pieces) store_piece1(d.out_fmt, d.outp, idx + e, w[e]);
+        for (int e = 0; e < 4; ++e) {
+          float x = ca * a4[e];
+          x = fmaf(cb, p4[e], x);
+          x = fmaf(cc, w[e], x);
+          if (half_id) x = (x + (i == j0 + e ? 1.0f : 0.0f)) * 0.5f;
+          w[e] = x;
         }
       }
-    }
-    if (poly || cscale != 1.0f) {  // pass 2 reads final values
+      if (!diag) {
+        if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
+        if constexpr (pieces) put4(idx, w);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) C[r * CS + c0 + e] = w[e];
-    }
+        for (int e = 0; e < 4; ++e) {
+          if (i <= j0 + e) {
+            if (c32) c32[idx + e] = w[e];
+            if constexpr (pieces) put1(idx + e, w[e]);
+          }
+        }
+      }
+      if (poly || cscale != 1.0f) {  // pass 2 reads final values
+#pragma unroll
+        for (int e = 0; e < 4; ++e) C[r * CS + c0 + e] = w[e];
+      }
     }
   }
   __syncthreads();
   // pass 2: mirror block, columns of the tile
+#pragma unroll 2
   for (int q = tid; q < BT * nq; q += NT) {
     const int c = q / nq, rr = (q % nq) * 4;
     const int j = col0 + c, i0 = row0 + rr;
@@ -267,17 +230,90 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
     for (int e = 0; e < 4; ++e) w[e] = C[(rr + e) * CS + c];
     if (!diag) {
       if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
-      if (pieces) store_pieces4(d.out_fmt, d.outp, idx, w);
+      if constexpr (pieces) put4(idx, w);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if (i0 + e < j) {
           if (c32) c32[idx + e] = w[e];
-          if (pieces) store_piece1(d.out_fmt, d.outp, idx + e, w[e]);
+          if constexpr (pieces) put1(idx + e, w[e]);
         }
       }
     }
   }
+}
+
+template <int BT, int NT = 256>
+__device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
+                                              float cscale = 1.0f) {
+  constexpr int CS = BT + 1;
+  constexpr int Q4 = BT / 4;
+  const int tid = threadIdx.x;
+  const int row0 = bi * BT + r0, col0 = bj * BT;
+  const bool diag = bi == bj;
+  const int64_t ld = d.ld;
+  const int nq = NR / 4;
+  // Every descriptor field is copied to a local before the first store: the
+  // descriptor may live in device memory (grouped launches), where the
+  // compiler would otherwise reload it after each store it cannot prove
+  // disjoint from it.
+  auto c32 = gbl(d.c32);
+  auto p3 = gbl(d.p3);
+  const int mode = d.mode, out_fmt = d.out_fmt, n = d.n, out_f64 = d.out_f64;
+  const float ca = d.a, cb = d.b, cc = d.c;
+  uint16_t* const outp[3] = {d.outp[0], d.outp[1], d.outp[2]};
+  if (mode == EPI_OUTPUT) {
+    auto out64 = gbl((double*)d.out);
+    auto out32 = gbl((float*)d.out);
+    const double s = (d.scale ? *d.scale : 1.0) * (double)cscale;
+    const int64_t old = d.out_ld;
+    // (i, j) at i + j*ld: contiguous along i
+#pragma unroll 2
+    for (int q = tid; q < BT * nq; q += NT) {
+      const int c = q / nq, rr = (q % nq) * 4;
+      const int j = col0 + c;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = row0 + rr + e;
+        if (i < n && j < n && (!diag || i <= j)) {
+          const double v = (double)C[(rr + e) * CS + c] * s;
+          if (out_f64) out64[(int64_t)i + (int64_t)j * old] = v;
+          else out32[(int64_t)i + (int64_t)j * old] = (float)v;
+        }
+      }
+    }
+    // (j, i) at j + i*ld: contiguous along j
+#pragma unroll 2
+    for (int q = tid; q < NR * Q4; q += NT) {
+      const int r = q / Q4, c0 = (q % Q4) * 4;
+      const int i = row0 + r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = col0 + c0 + e;
+        if (i < n && j < n && (!diag || i < j)) {
+          const double v = (double)C[r * CS + c0 + e] * s;
+          if (out_f64) out64[(int64_t)j + (int64_t)i * old] = v;
+          else out32[(int64_t)j + (int64_t)i * old] = (float)v;
+        }
+      }
+    }
+    return;
+  }
+  const bool poly = mode == EPI_POLY || mode == EPI_POLY_HALF_ID;
+  const bool half_id = mode == EPI_POLY_HALF_ID;
+#define PSDP_EPI(OF)                                                                                      \
+  epilogue_store_rows<BT, NT, OF>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id, ca, \
+                                  cb, cc)
+  switch (out_fmt) {
+    case FMT_F16: PSDP_EPI(FMT_F16); break;
+    case FMT_F16X3: PSDP_EPI(FMT_F16X3); break;
+    case FMT_BF16: PSDP_EPI(FMT_BF16); break;
+    case FMT_BF16X3: PSDP_EPI(FMT_BF16X3); break;
+    case FMT_BF16X9: PSDP_EPI(FMT_BF16X9); break;
+    case FMT_E4M3: PSDP_EPI(FMT_E4M3); break;
+    default: PSDP_EPI(FMT_F32); break;
+  }
+#undef PSDP_EPI
 }
 
 // MFMAs over one BK-wide LDS slab: acc += X-rows x Y-rows^T.
