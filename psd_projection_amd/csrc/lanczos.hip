@@ -43,24 +43,44 @@ __device__ __forceinline__ void gemv_body(const TA* __restrict__ A, int64_t ld, 
   for (int r = 0; r < GEMV_RPW; ++r) acc[r] = 0.0;
   const auto Ag = gbl(A);
   const auto xg = gbl(x);
-  for (int k = lane * VEC; k < m; k += 64 * VEC) {
-    double xv[VEC];
+  // KU chunks of 64 VEC columns per pass: all their loads are issued before
+  // the first FMA, so a wave keeps KU x RPW 16-byte loads in flight.  Each
+  // accumulator still sees its terms in ascending k (the sum is the same
+  // for any KU).
+  constexpr int KU = 4;
+  for (int k0 = lane * VEC; k0 < m; k0 += 64 * VEC * KU) {
+    TA av[KU][GEMV_RPW][VEC];
+    double xv[KU][VEC];
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) xv[e] = xg[k + e];
+    for (int u = 0; u < KU; ++u) {
+      const int k = k0 + u * 64 * VEC;
+      if (k < m) {
 #pragma unroll
-    for (int r = 0; r < GEMV_RPW; ++r) {
-      if (r0 + r < m) {
-        const auto row = Ag + (int64_t)(r0 + r) * ld + k;
-        TA av[VEC];
-        if constexpr (VEC * sizeof(TA) == 16) {
-          typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
-          *(u32x4v*)av = *(const __attribute__((address_space(1))) u32x4v*)row;
-        } else {
+        for (int e = 0; e < VEC; ++e) xv[u][e] = xg[k + e];
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) av[e] = row[e];
+        for (int r = 0; r < GEMV_RPW; ++r) {
+          if (r0 + r < m) {
+            const auto row = Ag + (int64_t)(r0 + r) * ld + k;
+            if constexpr (VEC * sizeof(TA) == 16) {
+              typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+              *(u32x4v*)av[u][r] = *(const __attribute__((address_space(1))) u32x4v*)row;
+            } else {
+#pragma unroll
+              for (int e = 0; e < VEC; ++e) av[u][r][e] = row[e];
+            }
+          }
         }
+      }
+    }
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[r] = fma((double)av[e], xv[e], acc[r]);
+    for (int u = 0; u < KU; ++u) {
+      if (k0 + u * 64 * VEC < m) {
+#pragma unroll
+        for (int r = 0; r < GEMV_RPW; ++r)
+          if (r0 + r < m) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[r] = fma((double)av[u][r][e], xv[u][e], acc[r]);
+          }
       }
     }
   }
