@@ -174,6 +174,15 @@ typedef struct psdp_block {
 } psdp_block_t;
 psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* blocks, int count,
                                        const psdp_config_t* cfg);
+/* Packed block-diagonal cone, the on-device form an ADMM/SDP solver keeps:
+ * count column-major blocks of sizes[b] x sizes[b] floats stored back to back
+ * (block b starts at element sum_{c<b} sizes[c]^2) in one device buffer X;
+ * the projected blocks go to Y in the same layout (Y == X projects in place).
+ * Same arithmetic as psdp_project_batched_f32 on those blocks.  No reference
+ * counterpart: the reference projects one matrix per call (its intended
+ * caller, an SDP solver, include/psd_projection/check.h:10, loops over cones). */
+psdp_status_t psdp_project_packed_f32(psdp_handle_t h, const float* X, float* Y, const int64_t* sizes, int count,
+                                      const psdp_config_t* cfg);
 /* Tile classes of the batched call: blocks with n >= big_block_n use
  * 128 x 128 tiles, the rest 64 x 64 (0: the built-in default); concurrent != 0
  * runs the two classes of each product on two streams. */

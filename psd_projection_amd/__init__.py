@@ -113,6 +113,8 @@ _PROTOS = {
                                             ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "psdp_project_batched_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Block), ctypes.c_int,
                                                 ctypes.POINTER(Config)]),
+    "psdp_project_packed_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.POINTER(Config)]),
     "psdp_workspace_size_batched": (ctypes.c_int, [ctypes.POINTER(Block), ctypes.c_int, ctypes.POINTER(Config),
                                                    ctypes.POINTER(ctypes.c_size_t)]),
     "psdp_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -568,11 +570,32 @@ def project_batched(handle, blocks, outs=None, method="mixed", config: Config | 
     return outs
 
 
+def project_packed(handle, data, sizes, out=None, method="mixed", config: Config | None = None, **overrides):
+    """Projects a packed block-diagonal cone: `data` is a 1-D float32 device
+    tensor holding column-major blocks of the given sizes back to back
+    (block b starts at sum of sizes[c]**2 for c < b).  Writes `out` (default:
+    `data`, in place) in the same layout and returns it."""
+    sizes = [int(v) for v in sizes]
+    if data.dim() != 1 or not data.is_contiguous():
+        raise ValueError("packed data must be a contiguous 1-D tensor")
+    if sum(v * v for v in sizes) != data.numel():
+        raise ValueError(f"sizes cover {sum(v * v for v in sizes)} elements, data holds {data.numel()}")
+    if out is None:
+        out = data
+    elif out.shape != data.shape or not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor shaped like data")
+    handle._sync_stream()
+    arr = (ctypes.c_int64 * len(sizes))(*sizes)
+    cfg = config if config is not None else default_config(method, **overrides)
+    _check(lib().psdp_project_packed_f32(handle.ptr, _ptr(data), _ptr(out), arr, len(sizes), ctypes.byref(cfg)))
+    return out
+
+
 __all__ = [
     "Handle", "PsdpError", "Config", "default_config", "lib",
     "composite_FP32", "composite_FP32_auto_scale", "composite_FP16", "composite_FP16_auto_scale",
     "composite_FP32_emulated", "composite_FP32_emulated_auto_scale", "approximate_two_norm",
-    "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "sym_product", "workspace_size",
+    "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "project_packed", "sym_product", "workspace_size",
     "compute_eigenpairs", "lobpcg", "composite_FP32_auto_scale_deflate", "composite_FP16_auto_scale_deflate",
     "eig_FP64_deflate",
 ]

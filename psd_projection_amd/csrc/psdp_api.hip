@@ -1565,6 +1565,21 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   return PSDP_OK;
 }
 
+psdp_status_t psdp_project_packed_f32(psdp_handle_t h, const float* X, float* Y, const int64_t* sizes, int count,
+                                      const psdp_config_t* cfg) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (!X || !Y || !sizes) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (count <= 0 || count > 65535) return fail(PSDP_ERR_INVALID_ARG, "block count must be in [1, 65535]");
+  std::vector<psdp_block_t> blocks((size_t)count);
+  int64_t off = 0;
+  for (int b = 0; b < count; ++b) {
+    if (sizes[b] <= 0 || sizes[b] > 65536) return fail(PSDP_ERR_INVALID_ARG, "block %d: bad size", b);
+    blocks[b] = psdp_block_t{X + off, Y + off, sizes[b]};
+    off += sizes[b] * sizes[b];
+  }
+  return psdp_project_batched_f32(h, blocks.data(), count, cfg);
+}
+
 psdp_status_t psdp_autotune_batched(psdp_handle_t h, const psdp_block_t* blocks, int count,
                                     const psdp_config_t* cfg) {
   if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");

@@ -329,6 +329,33 @@ def test_batched_rejects_bad_blocks(handle):
         P.project_batched(handle, [])
 
 
+def test_packed_cone_equals_block_list(handle):
+    """The packed block-diagonal layout (blocks back to back in one buffer,
+    projected in place or out of place) gives bitwise the block-list result."""
+    data = batched_inputs(12, 6, lo=20, hi=400)
+    Xs = [to_dev(X, torch.float32) for X, _ in data]
+    sizes = [x.shape[0] for x in Xs]
+    handle.set_seed(5)
+    ref = P.project_batched(handle, Xs)
+    packed = torch.cat([x.reshape(-1) for x in Xs])
+    keep = packed.clone()
+    handle.set_seed(5)
+    out = P.project_packed(handle, packed, sizes, out=torch.empty_like(packed))
+    assert torch.equal(packed, keep)  # out of place leaves the input alone
+    handle.set_seed(5)
+    inplace = P.project_packed(handle, packed, sizes)
+    assert inplace.data_ptr() == packed.data_ptr()
+    off = 0
+    for n, y in zip(sizes, ref):
+        assert torch.equal(out[off:off + n * n].view(n, n), y)
+        assert torch.equal(packed[off:off + n * n].view(n, n), y)
+        off += n * n
+    with pytest.raises(ValueError):
+        P.project_packed(handle, packed, sizes[:-1])
+    with pytest.raises(P.PsdpError):
+        P.project_packed(handle, packed[:4], [0, 2])
+
+
 # ------------------------------------------------------------------ EVD drop-ins
 def test_eig_fp64_psd(handle):
     X, Q, lam = O.spectral_input(300, 21, "normal")
