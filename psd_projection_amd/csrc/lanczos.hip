@@ -15,11 +15,14 @@
 //     w1, so its residual is A y - theta y), lo = sqrt(theta) (:264),
 //     up = sqrt(theta + ||r||) (:270); nb_iter == 0 gives lo = 0, up = 1
 //     (:171-184).
-// The matrix-vector products are a bandwidth-bound GEMV (one wave streams
-// RPW rows with 16-byte loads, FP64 accumulation, wave-shuffle reduction);
-// vector updates run in one 1024-thread workgroup per matrix.  Every kernel
-// has a single-matrix form and a batched form (one launch for many matrices,
-// blockIdx.y or blockIdx.x selects the matrix) sharing the same body, so a
+// The matrix-vector products are bandwidth-bound.  On the float32 matrix of
+// the projections they read only the tiles of the upper triangle (half the
+// bytes of a full GEMV; FP64 accumulation, wave-shuffle reductions, partial
+// sums combined in a fixed order by a second kernel); the float64 drop-in
+// streams whole rows (one wave per RPW rows, 16-byte loads).  Vector updates
+// run in one 1024-thread workgroup per matrix.  Every kernel has a
+// single-matrix form and a batched form (one launch for many matrices, a flat
+// work list or blockIdx.y selects the matrix) sharing the same body, so a
 // matrix gets bitwise the same bound alone and in a batch.
 #include "psdp_device.h"
 
@@ -27,7 +30,6 @@ namespace psdp {
 
 constexpr int GEMV_RPW = 4;                    // rows per wave
 constexpr int GEMV_ROWS_PER_BLOCK = 4 * GEMV_RPW;
-static_assert(GEMV_ROWS_PER_BLOCK == GEMV_BATCH_ROWS, "batched GEMV row groups");
 
 // ------------------------------------------------------------------ GEMV
 // y[r] = sum_k A[r*ld + k] x[k] for the rows of this workgroup (A symmetric:
@@ -99,17 +101,173 @@ __global__ __launch_bounds__(256) void gemv_sym_kernel(const TA* __restrict__ A,
   gemv_body<TA, VEC>(A, ld, m, x, y, blockIdx.x);
 }
 
+// ------------------------------------------------------------------ symmetric GEMV
+// y = A x for a symmetric float32 m x m matrix (m % 4 == 0, ld % 4 == 0,
+// 16-byte aligned rows) that reads only the upper triangle's tiles.  Tile
+// (I, J), I <= J, of T x T entries (T = SYMV_TILE, clipped at m) contributes
+//   A_IJ x_J   to the rows of block I:  P[J][I T + r]
+//   A_IJ^T x_I to the rows of block J:  P[I][J T + c]   (I < J only)
+// so every P[t][i] (t < ceil(m / T), i < m) is written exactly once and
+// y[i] = sum_t P[t][i] in ascending t.  All sums have a fixed order: y is
+// deterministic and the same for any number of zero rows padding the matrix.
+//
+// In a tile, wave w takes rows w + 4 j (64 rows), lane l columns 4l .. 4l+3
+// (one 16-byte load per row).  A row's direct sum is spread over the 64
+// lanes: eight rows at a time are reduced by a butterfly that halves the
+// values each lane carries at every exchange (4 + 2 + 1 + 3 shuffles for 8
+// rows instead of 6 per row).  The transposed sums stay in the lane
+// (its 4 columns over the wave's rows, x of the rows broadcast from LDS) and
+// the 4 waves combine them in LDS.
+__device__ __forceinline__ void symv_tile_body(const float* __restrict__ A, int64_t ld, int m,
+                                               const double* __restrict__ x, double* __restrict__ P, int I,
+                                               int J) {
+  constexpr int T = SYMV_TILE;
+  static_assert(T == 256, "4 waves x 64 rows, 64 lanes x 4 columns");
+  typedef float f32x4v __attribute__((ext_vector_type(4)));
+  __shared__ double red[4 * T];
+  __shared__ double xs[T];  // x of the tile's rows
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = I * T, c0 = J * T;
+  const int c = c0 + 4 * lane;
+  const bool cin = c < m;  // m % 4 == 0: the lane's 4 columns are all inside or all outside
+  const auto xg = gbl(x);
+  const auto Ag = gbl(A);
+  {
+    const int r = r0 + (int)threadIdx.x;
+    xs[threadIdx.x] = r < m ? xg[r] : 0.0;
+  }
+  double xc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) xc[e] = cin ? xg[c + e] : 0.0;
+  __syncthreads();
+  double z[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0;
+  const int kl = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);  // row of the batch this lane ends with
+
+  // batch b: rows w + 4 (8 b + k), k < 8
+  auto load = [&](int b, f32x4v (&dst)[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int r = r0 + w + 4 * (8 * b + k);
+      dst[k] = (cin && r < m) ? *(const __attribute__((address_space(1))) f32x4v*)(Ag + (int64_t)r * ld + c)
+                              : f32x4v{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  auto process = [&](int b, const f32x4v (&cur)[8]) {
+    double p[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double a0 = cur[k].x, a1 = cur[k].y, a2 = cur[k].z, a3 = cur[k].w;
+      double sk = a0 * xc[0];
+      sk = fma(a1, xc[1], sk);
+      sk = fma(a2, xc[2], sk);
+      sk = fma(a3, xc[3], sk);
+      p[k] = sk;
+      const double xr = xs[w + 4 * (8 * b + k)];  // same address in every lane: broadcast
+      z[0] = fma(a0, xr, z[0]);
+      z[1] = fma(a1, xr, z[1]);
+      z[2] = fma(a2, xr, z[2]);
+      z[3] = fma(a3, xr, z[3]);
+    }
+    // butterfly: after the exchanges across 32, 16 and 8 lanes, lane l holds
+    // the partial of row kl = 4 b5 + 2 b4 + b3 (bits of l) over 8 lanes
+    double q4[4], q2[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double send = h5 ? p[j] : p[j + 4];
+      const double keep = h5 ? p[j + 4] : p[j];
+      q4[j] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double send = h4 ? q4[j] : q4[j + 2];
+      const double keep = h4 ? q4[j + 2] : q4[j];
+      q2[j] = keep + __shfl_xor(send, 16);
+    }
+    const double send = h3 ? q2[0] : q2[1];
+    const double keep = h3 ? q2[1] : q2[0];
+    double t = keep + __shfl_xor(send, 8);
+    t += __shfl_xor(t, 4);
+    t += __shfl_xor(t, 2);
+    t += __shfl_xor(t, 1);
+    const int r = r0 + w + 4 * (8 * b + kl);
+    if ((lane & 7) == 0 && r < m) gbl(P)[(int64_t)J * m + r] = t;
+  };
+  f32x4v a0[8], a1[8];
+  load(0, a0);
+#pragma unroll 1
+  for (int b = 0; b < 8; b += 2) {  // the loads of the next batch are in flight during each batch's FMAs
+    load(b + 1, a1);
+    process(b, a0);
+    if (b + 2 < 8) load(b + 2, a0);
+    process(b + 1, a1);
+  }
+  if (I != J) {  // uniform over the workgroup
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[w * T + 4 * lane + e] = z[e];
+    __syncthreads();
+    const int i = threadIdx.x;
+    const int col = c0 + i;
+    if (col < m) gbl(P)[(int64_t)I * m + col] = ((red[i] + red[T + i]) + red[2 * T + i]) + red[3 * T + i];
+  }
+}
+
+// y[i] = sum_t P[t][i] for the SYMV_TILE rows of one chunk.
+__device__ __forceinline__ void symv_reduce_body(const double* __restrict__ P, int m, double* __restrict__ y,
+                                                 int chunk) {
+  const int i = chunk * SYMV_TILE + (int)threadIdx.x;
+  if (i >= m) return;
+  const int nt = (m + SYMV_TILE - 1) / SYMV_TILE;
+  const auto Pg = gbl(P);
+  double s = 0.0;
+#pragma unroll 8
+  for (int t = 0; t < nt; ++t) s += Pg[(int64_t)t * m + i];
+  gbl(y)[i] = s;
+}
+
+// Tile L of the upper triangle in column order: (0,0), (0,1), (1,1), (0,2), ...
+__device__ __forceinline__ void tri_decode(int L, int& I, int& J) {
+  int j = (int)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+  while ((j + 1) * (j + 2) / 2 <= L) ++j;
+  while (j * (j + 1) / 2 > L) --j;
+  J = j;
+  I = L - j * (j + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void symv_tile_kernel(const float* __restrict__ A, int64_t ld, int m,
+                                                        const double* __restrict__ x, double* __restrict__ P,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  int I, J;
+  tri_decode((int)blockIdx.x, I, J);
+  symv_tile_body(A, ld, m, x, P, I, J);
+}
+
+__global__ __launch_bounds__(256) void symv_reduce_kernel(const double* __restrict__ P, int m,
+                                                          double* __restrict__ y, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  symv_reduce_body(P, m, y, (int)blockIdx.x);
+}
+
 // which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y (skip unless Ritz pair).
-// Workgroup g takes the GEMV_ROWS_PER_BLOCK rows rows[g].y of matrix rows[g].x
-// (a flat list over every matrix's row groups, so no workgroup idles).
-__global__ __launch_bounds__(256) void gemv_batched_kernel(const LanczosJob* __restrict__ jobs,
-                                                           const int2* __restrict__ rows, int which) {
-  const int2 rg = rows[blockIdx.x];
-  const LanczosJob& J = jobs[rg.x];
+// Flat work lists over every matrix: tiles[g] = {matrix, I, J}, chunks[g] =
+// {matrix, row chunk}, so no workgroup idles on a small matrix.
+__global__ __launch_bounds__(256) void symv_tile_batched(const LanczosJob* __restrict__ jobs,
+                                                         const TileRef* __restrict__ tiles, int which) {
+  const TileRef t = tiles[blockIdx.x];
+  const LanczosJob& J = jobs[t.mat];
   if (which < 2 ? J.st->done : J.st->skip_ritz) return;
   const double* x = which == 0 ? J.v.q : (which == 1 ? J.v.w1 : J.v.y);
+  symv_tile_body(J.A, J.ld, J.m, x, J.v.part, t.bi, t.bj);
+}
+
+__global__ __launch_bounds__(256) void symv_reduce_batched(const LanczosJob* __restrict__ jobs,
+                                                           const int2* __restrict__ chunks, int which) {
+  const int2 ch = chunks[blockIdx.x];
+  const LanczosJob& J = jobs[ch.x];
+  if (which < 2 ? J.st->done : J.st->skip_ritz) return;
   double* y = which == 0 ? J.v.w1 : (which == 1 ? J.v.w : J.v.ay);
-  gemv_body<float, 4>(J.A, J.ld, J.m, x, y, rg.y);
+  symv_reduce_body(J.v.part, J.m, y, ch.y);
 }
 
 // ------------------------------------------------------------------ start
@@ -429,19 +587,29 @@ static void launch_gemv(const TA* A, int64_t ld, int m, const double* x, double*
     hipLaunchKernelGGL((gemv_sym_kernel<TA, 1>), dim3(grid), dim3(256), 0, s, A, ld, m, x, y, skip);
 }
 
+// y = A x on the upper triangle of a float32 matrix (P: ceil(m/T) x m doubles).
+static void launch_symv(const float* A, int64_t ld, int m, const double* x, double* y, double* P, const int* skip,
+                        hipStream_t s) {
+  const int nt = (m + SYMV_TILE - 1) / SYMV_TILE;
+  hipLaunchKernelGGL(symv_tile_kernel, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, A, ld, m, x, P, skip);
+  hipLaunchKernelGGL(symv_reduce_kernel, dim3(nt), dim3(256), 0, s, P, m, y, skip);
+}
+
 // Enqueues the whole bound.  A: symmetric m x m (row-major view = column-major),
 // logical size n <= m (entries beyond n are zero).  Vectors have length m.
+// A float32 A needs m % 4 == 0, ld % 4 == 0, 16-byte alignment and v.part.
 hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
                           uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
                           hipStream_t s) {
+  if (!a_f64 && (m % 4 != 0 || ld % 4 != 0 || ((uintptr_t)A) % 16 != 0 || !v.part)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, s, st, v, n, m, seed);
   for (int k = 0; k < max_iter; ++k) {
     if (a_f64) {
       launch_gemv((const double*)A, ld, m, v.q, v.w1, &st->done, s);
       launch_gemv((const double*)A, ld, m, v.w1, v.w, &st->done, s);
     } else {
-      launch_gemv((const float*)A, ld, m, v.q, v.w1, &st->done, s);
-      launch_gemv((const float*)A, ld, m, v.w1, v.w, &st->done, s);
+      launch_symv((const float*)A, ld, m, v.q, v.w1, v.part, &st->done, s);
+      launch_symv((const float*)A, ld, m, v.w1, v.w, v.part, &st->done, s);
     }
     hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, s, st, v, k, m, max_iter, tol);
   }
@@ -450,27 +618,32 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
   if (a_f64)
     launch_gemv((const double*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
   else
-    launch_gemv((const float*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
+    launch_symv((const float*)A, ld, m, v.y, v.ay, v.part, &st->skip_ritz, s);
   hipLaunchKernelGGL(lanczos_bound_kernel, dim3(1), dim3(1024), 0, s, st, v, m, margin);
   return hipGetLastError();
 }
 
 // Batched bound over `count` float32 padded matrices (jobs in device memory;
-// every job's m is a multiple of 4 and at most max_m).
-// rows: one int2 {matrix, row group} per GEMV_ROWS_PER_BLOCK rows of every
-// matrix (nrows entries).
+// every job's m and ld are multiples of 4, m is at most max_m, each job has
+// its partial-sum buffer v.part).  tiles: {matrix, I, J} for every upper
+// triangle tile (I <= J < ceil(m / SYMV_TILE)) of every matrix; chunks:
+// {matrix, c} for every SYMV_TILE-row chunk c of every matrix.
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, const int2* rows, int nrows, hipStream_t s) {
-  const dim3 ggrid(nrows);
+                                  uint64_t seed, double margin, const TileRef* tiles, int ntiles,
+                                  const int2* chunks, int nchunks, hipStream_t s) {
+  const dim3 tgrid(ntiles), rgrid(nchunks);
   hipLaunchKernelGGL(lanczos_init_batched, dim3(count), dim3(1024), 0, s, jobs, seed);
   for (int k = 0; k < max_iter; ++k) {
-    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 0);
-    hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 1);
+    for (int which = 0; which < 2; ++which) {
+      hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, which);
+      hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, which);
+    }
     hipLaunchKernelGGL(lanczos_step_batched, dim3(count), dim3(1024), 0, s, jobs, k, tol);
   }
   hipLaunchKernelGGL(lanczos_ritz_batched, dim3(count), dim3(64), 0, s, jobs);
   hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
-  hipLaunchKernelGGL(gemv_batched_kernel, ggrid, dim3(256), 0, s, jobs, rows, 2);
+  hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, 2);
+  hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, 2);
   hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin);
   return hipGetLastError();
 }

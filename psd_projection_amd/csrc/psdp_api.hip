@@ -44,7 +44,8 @@ hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s);
 hipError_t launch_fill_sym(float* M, int64_t ld, int n, int npad, uint64_t seed, hipStream_t st);
 hipError_t launch_copy_words(void* dst, const void* src, int64_t words, hipStream_t st);
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, const int2* rows, int nrows, hipStream_t s);
+                                  uint64_t seed, double margin, const TileRef* tiles, int ntiles,
+                                  const int2* chunks, int nchunks, hipStream_t s);
 hipError_t launch_sym_pad_batched(const PadJob* jobs, const TileRef* tiles, int ntiles, hipStream_t st);
 hipError_t launch_scale_split_batched(const SplitJob* jobs, const int2* chunks, int nchunks, int fmt_a, int fmt_x,
                                       hipStream_t st);
@@ -541,6 +542,9 @@ static Bufs carve(const Plan& p, char* base) {
   b.v.y = (double*)(v + 4 * vb);
   b.v.ay = (double*)(v + 5 * vb);
   b.v.V = (double*)(v + 6 * vb);
+  // the Lanczos GEMV's partial sums (ceil(npad/256) x npad doubles, at most
+  // half of npad x ld floats) use A^3's buffer, which is written only later
+  b.v.part = (double*)b.p3;
   return b;
 }
 
@@ -1046,6 +1050,7 @@ static psdp_status_t two_norm_locked(psdp_handle_t h, const double* A, int64_t n
   lv.y = (double*)(v + 4 * vb);
   lv.ay = (double*)(v + 5 * vb);
   lv.V = (double*)(v + 6 * vb);
+  lv.part = nullptr;  // float64 matrix: full-read GEMV
   HIPCHK(launch_lanczos(A, 1, n, (int)n, (int)n, iters, tol, h->seed, 1.0, h->st, lv, h->stream));
   LanczosState st;
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1189,11 +1194,11 @@ struct BatchPlan {
   double tol = 1e-10, margin = 1.0;
   int count = 0, nstages = 0, max_npad = 0;
   int ntiles_big = 0, ntiles_small = 0;  // 128 x 128 tiles first, then 64 x 64
-  int npad_tiles = 0, ngemv_rows = 0, nsplit_chunks = 0;  // flat work lists of the prep kernels
+  int npad_tiles = 0, nsymv_tiles = 0, nsymv_chunks = 0, nsplit_chunks = 0;  // flat work lists
   int64_t max_N = 0;
   size_t off_states = 0, off_desc = 0, total = 0;
   size_t d_states = 0, d_lz = 0, d_pad = 0, d_split = 0, d_prod = 0, d_tiles = 0;
-  size_t d_padt = 0, d_rows = 0, d_chunks = 0;
+  size_t d_padt = 0, d_symt = 0, d_rows = 0, d_chunks = 0;
   std::vector<char> blob;  // initial states + descriptors, built for one workspace base
 };
 
@@ -1263,20 +1268,24 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   bp.ntiles_big = (int)tiles.size();
   bp.ntiles_small = (int)small.size();
   tiles.insert(tiles.end(), small.begin(), small.end());
-  // flat work lists: 64 x 64 tile pairs of the symmetrize/pad, row groups of
-  // the Lanczos GEMV, value chunks of the scale/split
-  std::vector<TileRef> padt;
+  // flat work lists: 64 x 64 tile pairs of the symmetrize/pad, upper-triangle
+  // tiles and row chunks of the Lanczos GEMV, value chunks of the scale/split
+  std::vector<TileRef> padt, symt;
   std::vector<int2> rows, chunks;
   for (int b = 0; b < count; ++b) {
     const int T = bb[b].npad / 64;
     for (int i = 0; i < T; ++i)
       for (int j = i; j < T; ++j) padt.push_back(TileRef{b, i, j, 0});
-    for (int r = 0; r < bb[b].npad / GEMV_BATCH_ROWS; ++r) rows.push_back(make_int2(b, r));
+    const int nt = (bb[b].npad + SYMV_TILE - 1) / SYMV_TILE;
+    for (int j = 0; j < nt; ++j)
+      for (int i = 0; i <= j; ++i) symt.push_back(TileRef{b, i, j, 0});
+    for (int r = 0; r < nt; ++r) rows.push_back(make_int2(b, r));
     const int64_t nc = (int64_t)bb[b].npad * bb[b].npad / SPLIT_CHUNK;
     for (int64_t c = 0; c < nc; ++c) chunks.push_back(make_int2(b, (int)c));
   }
   bp.npad_tiles = (int)padt.size();
-  bp.ngemv_rows = (int)rows.size();
+  bp.nsymv_tiles = (int)symt.size();
+  bp.nsymv_chunks = (int)rows.size();
   bp.nsplit_chunks = (int)chunks.size();
   bp.nstages = 3 * sch.nsteps + 1;
   bp.off_states = off;
@@ -1290,6 +1299,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   bp.d_prod = doff, doff += align256(sizeof(ProdDesc) * (size_t)bp.nstages * count);
   bp.d_tiles = doff, doff += align256(sizeof(TileRef) * tiles.size());
   bp.d_padt = doff, doff += align256(sizeof(TileRef) * padt.size());
+  bp.d_symt = doff, doff += align256(sizeof(TileRef) * symt.size());
   bp.d_rows = doff, doff += align256(sizeof(int2) * rows.size());
   bp.d_chunks = doff, doff += align256(sizeof(int2) * chunks.size());
   bp.total = off + doff;
@@ -1306,6 +1316,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   ProdDesc* pd = (ProdDesc*)(blob + bp.d_prod);
   memcpy(blob + bp.d_tiles, tiles.data(), sizeof(TileRef) * tiles.size());
   memcpy(blob + bp.d_padt, padt.data(), sizeof(TileRef) * padt.size());
+  memcpy(blob + bp.d_symt, symt.data(), sizeof(TileRef) * symt.size());
   memcpy(blob + bp.d_rows, rows.data(), sizeof(int2) * rows.size());
   memcpy(blob + bp.d_chunks, chunks.data(), sizeof(int2) * chunks.size());
 
@@ -1340,6 +1351,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     J.v.y = (double*)(v + 4 * vb);
     J.v.ay = (double*)(v + 5 * vb);
     J.v.V = (double*)(v + 6 * vb);
+    J.v.part = (double*)B.p3;  // GEMV partial sums; A^3 is written only after the bound
     pj[b] = PadJob{blocks[b].X, q.n, q.npad, B.a32};
     SplitJob& S = sj[b];
     S.a32 = B.a32;
@@ -1481,7 +1493,8 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
     if (cfg->auto_scale) {
       ProfScope ps(h, PSDP_PROF_LANCZOS);
       HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin,
-                                    (const int2*)(ddev + bp.d_rows), bp.ngemv_rows, s));
+                                    (const TileRef*)(ddev + bp.d_symt), bp.nsymv_tiles,
+                                    (const int2*)(ddev + bp.d_rows), bp.nsymv_chunks, s));
     }
     {
       ProfScope ps(h, PSDP_PROF_PREP);

@@ -61,9 +61,10 @@ struct TileRef {
 
 constexpr int LANCZOS_MAX_ITER = 64;
 
-// Rows per workgroup of the batched GEMV, values per workgroup of the batched
+// Tile edge of the symmetric float32 GEMV of the Lanczos bound (also the rows
+// per workgroup of its reduction), values per workgroup of the batched
 // scale/split (a 64 x 64 tile holds a whole number of chunks).
-constexpr int GEMV_BATCH_ROWS = 16;
+constexpr int SYMV_TILE = 256;
 constexpr int SPLIT_CHUNK = 4096;
 
 // Device-resident Lanczos state (one per matrix).
@@ -92,6 +93,7 @@ struct LanczosVecs {
   double* V;     // column j at V + j*m
   double* y;
   double* ay;
+  double* part;  // float32 GEMV partial sums: ceil(m / SYMV_TILE) x m (null: full-read GEMV only)
 };
 
 // One matrix of a batched Lanczos launch (float32 padded input).
