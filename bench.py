@@ -147,11 +147,12 @@ def bench_batched(args, P, W, torch):
         # element, each product's epilogue writes its result (pieces; float32
         # copies of A^3 and of the new A for the polynomial) and the polynomial
         # reads A and A^3 in float32, and each product reads its two operands
-        # once; 40 Lanczos GEMVs read the float32 matrix.
+        # once; 41 Lanczos GEMVs read the upper triangle of the float32 matrix.
         npad2 = sum(float(((n + 63) // 64 * 64) ** 2) for n in sizes)
+        lanczos_bytes = 41.0 * sum(symv_bytes((n + 63) // 64 * 64) for n in sizes)
         per_step = {"f16x3": 4 + (4 + 4) + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
         steps = ["f16x3"] * 4 + ["f16"] * 3
-        bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + npad2 * 4 * 41
+        bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + lanczos_bytes
         t_hbm = bytes_batch / 8.0e12 * 1e3
         t_mfma = flops / (PEAK_TFLOPS["f16"] * 1e12) * 1e3
         res["roofline"] = {"bound": "hbm" if t_hbm > t_mfma else "mfma", "model_ms_hbm": round(t_hbm, 3),
@@ -206,6 +207,83 @@ def bench_deflate(args, P, W, torch):
     return res
 
 
+SYMV_TILE = 256
+
+
+def symv_bytes(m):
+    """float32 bytes one Lanczos GEMV reads: the 256 x 256 tiles (clipped at m)
+    of the upper triangle of the padded m x m matrix."""
+    nt = (m + SYMV_TILE - 1) // SYMV_TILE
+    edge = [min(SYMV_TILE, m - t * SYMV_TILE) for t in range(nt)]
+    return 4 * sum(edge[i] * edge[j] for j in range(nt) for i in range(j + 1))
+
+
+def _timed(P, torch, h, X, Y, cfg, reps):
+    """Median ms of reps projections (each bracketed by events on torch's
+    current stream, which the handle uses)."""
+    P.project(h, X, out=Y, config=cfg)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        P.project(h, X, out=Y, config=cfg)
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def bench_small_configs(args, P, W, torch):
+    """Config 1 (n=512, lambda ~ U[-1,1], plain composite_FP32, also the EVD
+    projection on the host cores) and config 2 (n=8192, lambda ~ N(0,1) scaled
+    to max|lambda| = 10 with 30% in [-1e-2, 1e-2]: plain FP32 chain against the
+    reference's FP16 chain and the shipped mixed schedule).  Errors are
+    relative Frobenius errors against the exact projection Q max(L,0) Q^T."""
+    import numpy as np
+    from oracle import psdp_oracle as O
+    out = {}
+    h = P.Handle(0, seed=2001)
+    X64, Pm, _ = W.spectral_matrix(512, 1, "uniform")
+    X = X64.float().contiguous()
+    Y = torch.empty_like(X)
+    cfg = P.default_config("composite_FP32", synchronize=0)
+    ms = _timed(P, torch, h, X, Y, cfg, 21)
+    err = W.rel_fro_error(Y, Pm)
+    host = X64.cpu().numpy()
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.eig_FP64_psd(host)
+        cpu.append((time.perf_counter() - t0) * 1e3)
+    out["cfg1_n512"] = {"workload": "n=512, Haar Q (seed 1), lambda U[-1,1], float32, composite_FP32 auto-scale",
+                        "ms": round(ms, 3), "rel_err": err, "cpu_eig_fp64_ms": round(sorted(cpu)[1], 2)}
+    log(f"config 1: {out['cfg1_n512']}")
+    del X, Y, X64, Pm
+    X64, Pm, _ = W.spectral_matrix(8192, 2, "cfg2")
+    X = X64.float().contiguous()
+    del X64
+    Y = torch.empty_like(X)
+    res = {"workload": "n=8192, Haar Q (seed 2), lambda N(0,1) scaled to max|lambda|=10, 30% in [-1e-2,1e-2], "
+                       "float32 input"}
+    for m in ("composite_FP32", "composite_FP16", "mixed"):
+        cfg = P.default_config(m, synchronize=0)
+        if args.autotune:
+            h.autotune(8192, config=cfg)
+        ms = _timed(P, torch, h, X, Y, cfg, 3)
+        err = W.rel_fro_error(Y, Pm)
+        res[m] = {"ms": round(ms, 3), "rel_err": err, "meets_1e-3": bool(err <= 1e-3)}
+    res["fp16_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["composite_FP16"]["ms"], 2)
+    res["mixed_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["mixed"]["ms"], 2)
+    out["cfg2_n8192"] = res
+    log(f"config 2: {res}")
+    del X, Y, Pm
+    h.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_served(args, P, W, torch):
     """Config 5: Poisson stream of requests, n in {64,256,1024,4096,8192} with
     weights {.4,.3,.15,.1,.05}, lambda ~ N(0,1), FP32 in/out; admission by HBM
@@ -255,7 +333,7 @@ def bench_served(args, P, W, torch):
     torch.cuda.synchronize()
     st = latency_stats(done)
     e = torch.stack(errs).cpu().numpy() if errs else np.zeros(1)
-    res = {"workload": f"{len(reqs)} requests (full config: 10000), Poisson at {rate:.1f}/s = {args.serve_load} x "
+    res = {"workload": f"{len(reqs)} requests, Poisson at {rate:.1f}/s = {args.serve_load} x "
                        f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}",
            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
            "proj_per_s": round(st["proj_per_s"], 1), "capacity_proj_per_s": round(cap, 1),
@@ -277,8 +355,10 @@ def main():
     ap.add_argument("--precision-study", type=int, default=1,
                     help="config 3 variants (BF16x3, BF16x9, e4m3 first half + bf16) at the same size (0: skip)")
     ap.add_argument("--batched", type=int, default=256, help="config 4 block count (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=2000,
-                    help="config 5 requests in the timed Poisson stream (0: skip; the full config is 10000)")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="config 5 requests in the timed Poisson stream (0: skip; the config is 10000)")
+    ap.add_argument("--small-configs", type=int, default=1,
+                    help="configs 1 (n=512) and 2 (n=8192, FP32 vs FP16 chain) (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
     ap.add_argument("--deflate", type=int, default=1024, help="size of the deflated-projection leg (0: skip)")
     ap.add_argument("--autotune", type=int, default=1, help="tune tile sizes before timing (0: built-in tiles)")
@@ -384,6 +464,18 @@ def main():
            "chain_tflops": round(total_chain_flops / (chain_ms * 1e-3) / 1e12, 1),
            "kernels_ms": {k: round(v[1] / args.steps, 3) for k, v in prof.items()},
            "roofline": roofline}
+    if "lanczos" in prof and iters > 0:
+        # the whole bound (GEMV tiles, their partial-sum reductions and the
+        # vector steps) against the matrix bytes its 2 iters + 1 GEMVs must read
+        lz_ms = prof["lanczos"][1] / prof["lanczos"][0]
+        m = (n + 255) // 256 * 256  # padded size of the 16-bit schedules from n = 2048 (256 x 256 tiles)
+        gbytes = (2 * iters + 1) * symv_bytes(m)
+        out["lanczos_roofline"] = {"bound": "hbm", "achieved": round(gbytes / (lz_ms * 1e-3) / 1e9, 1),
+                                   "peak": 8000.0, "unit": "GB/s",
+                                   "frac": round(gbytes / (lz_ms * 1e-3) / 8e12, 4), "ms": round(lz_ms, 3),
+                                   "bytes": gbytes,
+                                   "note": "symmetric GEMV reading the upper triangle's 256x256 tiles of the "
+                                           "float32 matrix, FP64 accumulation; time of the whole bound"}
 
     if args.fp32_steps > 0:
         cfg32 = P.default_config("composite_FP32", synchronize=0)
@@ -406,6 +498,8 @@ def main():
 
     del X, Y, Pm
     torch.cuda.empty_cache()
+    if args.small_configs:
+        out["configs_1_2"] = bench_small_configs(args, P, W, torch)
     if args.batched > 0:
         out["batched"] = bench_batched(args, P, W, torch)
     if args.serve_requests > 0:
