@@ -439,7 +439,7 @@ def main():
                 "traffic": measured_traffic(n, fmt),
                 "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel, autotuned tiles "
                           f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds + the last partial "
-                          "round as 128x128 tiles in a second launch)",
+                          "round as 64x64 tiles in a second launch)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
                 "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
     total_chain_flops = sum(PRODUCTS[f] * wsym * c for f, (c, _) in prods.items()) / args.steps

@@ -226,7 +226,7 @@ struct psdp_handle_st {
   hipGraphExec_t bgraph = nullptr;
   std::vector<int64_t> bgraph_key;
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
-  std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, 128-tile tail)
+  std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, TAIL_SUB tail)
   int cus = 256;                                  // compute units of the device
   // pinned host staging ring for batched descriptors: slot k is reused only
   // after the copy recorded in ev[k] has run
@@ -314,10 +314,15 @@ static psdp_status_t get_tiles(psdp_handle_t h, int T, int bt, TileList& tl) {
 }
 
 // 256 x 256 tiles in whole rounds over the device's CUs, plus the last,
-// partial round re-cut into 128 x 128 tiles for a second launch: one tile per
-// CU at a time, so a partial round of big tiles (2080 tiles = 8.1 rounds at
-// n = 16384 on 256 CUs) would cost a whole round; as small tiles it costs
-// about a quarter.  Only when the partial round is at most half the CUs.
+// partial round re-cut into 64 x 64 tiles for a second launch: one big tile
+// per CU at a time, so a partial round of big tiles (2080 tiles = 8.1 rounds
+// at n = 16384 on 256 CUs) would cost a whole round.  Re-cut, the 32 leftover
+// tiles become 512 small ones that occupy every CU (two workgroups each);
+// measured at n = 16384 a binary16x3 product takes 12.14 ms with the 64 x 64
+// tail against 12.37 ms with a 128 x 128 tail (which fills only half the
+// CUs).  Only when the partial round is at most half the CUs.  Every launch
+// of the tail list uses TAIL_SUB.
+constexpr int TAIL_SUB = 2;  // 64 x 64
 static psdp_status_t get_split_tiles(psdp_handle_t h, int T, TileList& big, TileList& small) {
   auto it = h->split_tiles.find(T);
   if (it != h->split_tiles.end()) {
@@ -334,11 +339,12 @@ static psdp_status_t get_split_tiles(psdp_handle_t h, int T, TileList& big, Tile
   } else {
     bigv = xcd_interleave(std::vector<TileRef>(ord.begin(), ord.begin() + full));
     std::vector<TileRef> tail;
+    constexpr int K = 8 / TAIL_SUB;  // tail tiles per big tile edge
     for (int t = full; t < total; ++t)
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
-          const int r = 2 * ord[t].bi + a, c = 2 * ord[t].bj + b;
-          if (r <= c) tail.push_back(TileRef{0, r, c, 0});  // the lower quarter of a diagonal tile is a mirror
+      for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) {
+          const int r = K * ord[t].bi + a, c = K * ord[t].bj + b;
+          if (r <= c) tail.push_back(TileRef{0, r, c, 0});  // the lower part of a diagonal tile is a mirror
         }
     smallv = xcd_interleave(tail);
   }
@@ -575,7 +581,7 @@ static psdp_status_t launch_product(psdp_handle_t h, const Plan& p, int fmt, int
     TileList big, small;
     STCHK(get_split_tiles(h, p.npad / 256, big, small));
     HIPCHK(launch_sym_product(fmt, 8, avg, d, big.dev, big.count, s));
-    if (small.count) HIPCHK(launch_sym_product(fmt, 4, avg, d, small.dev, small.count, s));
+    if (small.count) HIPCHK(launch_sym_product(fmt, TAIL_SUB, avg, d, small.dev, small.count, s));
     return PSDP_OK;
   }
   TileList tl;
@@ -1131,7 +1137,7 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   {
     ProfScope ps(h, PSDP_PROF_PROD_F32 + format);
     HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
-    if (tail.count) HIPCHK(launch_sym_product(format, 4, average ? 1 : 0, d, tail.dev, tail.count, s));
+    if (tail.count) HIPCHK(launch_sym_product(format, TAIL_SUB, average ? 1 : 0, d, tail.dev, tail.count, s));
   }
   HIPCHK(hipStreamSynchronize(s));
   return PSDP_OK;

@@ -61,7 +61,7 @@ def test_sym_product(handle, fmt, avg, n):
 @pytest.mark.parametrize("fmt", ["f16", "f16x3"])
 def test_sym_product_split_tail(handle, fmt):
     # n = 6144: 300 tiles of 256 x 256, i.e. one round on 256 CUs plus a partial
-    # one that runs as 128 x 128 tiles in a second launch (on devices with
+    # one that runs as 64 x 64 tiles in a second launch (on devices with
     # fewer CUs the split differs; the result must not).
     # The built-in choice at this size is 64 x 64 tiles; 256 x 256 is set by
     # hand, and both must give the same bits.
