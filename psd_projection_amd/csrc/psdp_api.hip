@@ -270,8 +270,9 @@ static psdp_status_t staging_release(psdp_handle_t h) {
 // chunk of that order (its L2 then serves neighbouring panels).  G = 8 for
 // 256 x 256 tiles: an XCD's 32 CUs then run 4 x 8 tiles of one super-tile (12
 // distinct panels for 32 tiles instead of 16 with two 4 x 4 super-tiles);
-// measured at n = 16384 a binary16x3 product takes 12.52 ms against 12.76 ms
-// with G = 4.  Smaller tiles keep G = 4.
+// measured at n = 16384 (alternating runs on one device) a binary16x3 product
+// takes 12.04-12.07 ms against 12.15-12.16 ms with G = 4.  Smaller tiles keep
+// G = 4.
 static std::vector<TileRef> super_tile_order(int T, int G = 4) {
   std::vector<TileRef> ord;
   ord.reserve((size_t)T * (T + 1) / 2);
