@@ -351,7 +351,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--method", default="mixed")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=6144, help="CPU baseline block size (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=8192, help="CPU baseline block size (0: skip)")
     ap.add_argument("--precision-study", type=int, default=1,
                     help="config 3 variants (BF16x3, BF16x9, e4m3 first half + bf16) at the same size (0: skip)")
     ap.add_argument("--batched", type=int, default=256, help="config 4 block count (0: skip)")
