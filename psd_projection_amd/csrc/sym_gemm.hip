@@ -18,7 +18,8 @@
 // Tiling: 256 threads = 2 x 2 waves; each wave owns a (16*SUB) x (16*SUB)
 // sub-block made of 16x16 MFMA tiles (SUB = 8 gives 256 x 256 block tiles
 // with 128 x 128 per wave, the shape used for large n; SUB = 4 and 2 give
-// 128 and 64).  K is staged through LDS in BK-wide slabs, double buffered
+// 128 and 64).  The two-piece formats run their 256 x 256 tiles on 8 waves of
+// 128 x 64 each instead (sym_tile_dma8).  K is staged through LDS in BK-wide slabs, double buffered
 // (one barrier per slab).  Rows are padded by 16 bytes so that the 16-byte
 // fragment reads of 16 consecutive rows hit distinct banks.  Split formats
 // (x3, x9) accumulate every partial product into one accumulator.
@@ -128,7 +129,7 @@ constexpr int NR_MAX = BT < 128 ? BT : 128;
 // operand format OF fixed at compile time, so that the per-element split is
 // straight-line code and the loops stay compact (the instruction cache holds
 // them; a runtime format switch unrolled per element did not fit).
-template <int BT, int NT, int OF>
+template <int BT, int NT, int OF, int GMAX = 8>
 __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, float cscale, int row0, int col0,
                                                     bool diag, int64_t ld,
                                                     __attribute__((address_space(1))) float* c32,
@@ -163,7 +164,8 @@ __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, fl
   // pass 1: primary block, rows of the tile.  The polynomial's operands
   // (A and A^3 of this block) are loaded G quads ahead, so that a thread has
   // G loads in flight instead of waiting out each one before its stores.
-  constexpr int G = (NR_MAX<BT> * Q4 / NT) < 8 ? (NR_MAX<BT> * Q4 / NT > 0 ? NR_MAX<BT> * Q4 / NT : 1) : 8;
+  constexpr int G0 = (NR_MAX<BT> * Q4 / NT) < 8 ? (NR_MAX<BT> * Q4 / NT > 0 ? NR_MAX<BT> * Q4 / NT : 1) : 8;
+  constexpr int G = G0 < GMAX ? G0 : GMAX;
 #pragma unroll 1
   for (int q0 = tid; q0 < NR * Q4; q0 += NT * G) {
     f32x4 a4g[G], p4g[G];
@@ -243,7 +245,7 @@ __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, fl
   }
 }
 
-template <int BT, int NT = 256>
+template <int BT, int NT = 256, int GMAX = 8>
 __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
                                               float cscale = 1.0f) {
   constexpr int CS = BT + 1;
@@ -302,7 +304,7 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const bool poly = mode == EPI_POLY || mode == EPI_POLY_HALF_ID;
   const bool half_id = mode == EPI_POLY_HALF_ID;
 #define PSDP_EPI(OF)                                                                                      \
-  epilogue_store_rows<BT, NT, OF>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id, ca, \
+  epilogue_store_rows<BT, NT, OF, GMAX>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id, ca, \
                                   cb, cc)
   switch (out_fmt) {
     case FMT_F16: PSDP_EPI(FMT_F16); break;
@@ -771,6 +773,167 @@ __global__ __launch_bounds__(WS ? 512 : 256) void sym_prod_dma_kernel(ProdDesc d
   sym_tile_dma<FMT, SUB, WS>(d, t.bi, t.bj, smem);
 }
 
+// 256 x 256 tile with 8 waves (two per SIMD): wave w owns rows wr*128 .. +128
+// (wr = w >> 2) and columns wc*64 .. +64 (wc = w & 3), 4 x 2 blocks of 32 x 32
+// in 128 accumulator registers, so two waves share a SIMD and one wave's load
+// issue and barrier waits overlap the other's MFMAs.  Same slabs, stages,
+// swizzles and per-entry k order as sym_tile_dma<FMT, 8> (bitwise the same
+// results); every wave issues its share of the global->LDS loads.  The
+// fragments of one slab (all k-steps) live in registers; those of the next
+// slab replace them k-step by k-step once that k-step's MFMAs are issued.
+// The epilogue preloads at most 2 quads per thread (256 registers per wave).
+template <int FMT>
+__device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, int bi, int bj, char* smem) {
+  using DT = DmaTiling<FMT, 8, false>;
+  constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, S = DT::S;
+  constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
+  constexpr int NT = 512;
+  constexpr int L = SB / (NT * 16);  // 1 KB loads per thread per stage
+  static_assert(SB % (NT * 16) == 0 && L >= 1, "stage must split over 8 waves");
+  static_assert((S - 1) * L <= 63, "vmcnt range");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 2, wc = wave & 3;
+  const int64_t ld = d.ld;
+  const int row0 = bi * 256, col0 = bj * 256;
+
+  constexpr int CPR = RB / 16, RPB = 1024 / RB;
+  const uint16_t* gsrc[L];
+  int lofs[L], loff[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int q = wave * L + i;
+    const int gp = q / DT::BLOCKS, blk = q % DT::BLOCKS;
+    const int g = gp / P, p = gp % P;
+    const void* bx = p == 0 ? d.x[0] : (p == 1 ? d.x[1] : d.x[2]);
+    const void* by = p == 0 ? d.y[0] : (p == 1 ? d.y[1] : d.y[2]);
+    const void* base = g == 0 ? bx : by;
+    gsrc[i] = (const uint16_t*)base + (int64_t)((g == 0 ? row0 : col0) + blk * RPB) * ld;
+    lofs[i] = gp * TB + blk * 1024;
+    const int r = blk * RPB + lane / CPR;
+    loff[i] = (lane / CPR) * (int)ld + ((lane % CPR) ^ dma_swz<RB>(r)) * 8;
+  }
+#define PSDP_DMA8_ISSUE(K0, BUF)                                                                     \
+  _Pragma("unroll") for (int i = 0; i < L; ++i)                                                     \
+      __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (K0) + loff[i]),                     \
+                                       (lds_ptr_t)(smem + (BUF) * SB + lofs[i]), 16, 0, 0);
+
+  constexpr int MA = 4, MB = 2, KS = BK / 16;
+  f32x16 acc[MA][MB];
+#pragma unroll
+  for (int a = 0; a < MA; ++a)
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
+
+  const int arow = (wr * 128 + (lane & 31)) * RB;
+  const int brow = (wc * 64 + (lane & 31)) * RB;
+  const int swzl = dma_swz<RB>(lane & 31);
+  const int KT = d.npad / BK;
+
+  u32x4 fa[KS][P][MA], fb[KS][P][MB];
+  auto frag_a = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + p * TB + arow + m * 32 * RB + 16 * ((ks * 2 + (lane >> 5)) ^ swzl));
+  };
+  auto frag_b = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + (P + p) * TB + brow + m * 32 * RB + 16 * ((ks * 2 + (lane >> 5)) ^ swzl));
+  };
+
+#pragma unroll
+  for (int st = 0; st < S; ++st) { PSDP_DMA8_ISSUE((st < KT ? st : KT - 1) * BK, st) }
+  wait_vmcnt<(S - 1) * L>();
+  lds_barrier();
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+#pragma unroll
+      for (int m = 0; m < MA; ++m) fa[ks][p][m] = frag_a(smem, ks, p, m);
+#pragma unroll
+      for (int m = 0; m < MB; ++m) fb[ks][p][m] = frag_b(smem, ks, p, m);
+    }
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+
+#define PSDP_KSTEP_MMA(ks)                                                                        \
+  _Pragma("unroll") for (int mj = 0; mj < MB; ++mj)                                               \
+  _Pragma("unroll") for (int mi = 0; mi < MA; ++mi) {                                             \
+    f32x16 c = acc[mi][mj];                                                                       \
+    if constexpr (P == 2) { /* x3: lo*hi, hi*lo, then hi*hi */                                    \
+      c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                           \
+      c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                           \
+    }                                                                                             \
+    acc[mi][mj] = mfma32(fa[ks][0][mi], fb[ks][0][mj], c, FMT);                                   \
+  }
+
+  constexpr int NREAD = KS * P * (MA + MB);
+  constexpr int NMFMA = KS * MA * MB * (P == 1 ? 1 : 3);
+  constexpr int MR = NMFMA / NREAD > 0 ? NMFMA / NREAD : 1;
+  for (int kt = 0; kt + 1 < KT; ++kt) {
+    wait_vmcnt<(S - 2) * L>();          // own DMA of slab kt + 1 landed
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // own reads of buffer kt % S complete
+    lds_barrier();
+    PSDP_DMA8_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S)
+    const char* nxt = smem + ((kt + 1) % S) * SB;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      PSDP_KSTEP_MMA(ks)
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+#pragma unroll
+        for (int m = 0; m < MA; ++m) fa[ks][p][m] = frag_a(nxt, ks, p, m);
+#pragma unroll
+        for (int m = 0; m < MB; ++m) fb[ks][p][m] = frag_b(nxt, ks, p, m);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < NREAD; ++g) {
+      if (g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);            // DS read
+      __builtin_amdgcn_sched_group_barrier(0x008, MR, 0);            // MFMA
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) { PSDP_KSTEP_MMA(ks) }
+#undef PSDP_KSTEP_MMA
+#undef PSDP_DMA8_ISSUE
+  __builtin_amdgcn_sched_barrier(0);
+  wait_vmcnt<0>();
+  lds_barrier();
+
+  constexpr float PSCALE = (FMT == FMT_F16X3 || FMT == FMT_E4M3) ? F16X3_PROD_SCALE : 1.0f;
+  // Two phases; in phase ph every wave stages its row blocks 2ph and 2ph+1,
+  // i.e. tile rows wr*128 + ph*64 .. +64, at staged rows wr*64 .. +64, so that
+  // only half of the accumulators stay live while an epilogue runs.
+  float* Ct = (float*)smem;
+  constexpr int CS = 256 + 1;
+  // (phases written out: acc is indexed by compile-time constants only)
+#define PSDP_STAGE8(PH)                                                                           \
+  _Pragma("unroll") for (int h = 0; h < 2; ++h)                                                   \
+  _Pragma("unroll") for (int mj = 0; mj < MB; ++mj)                                               \
+  _Pragma("unroll") for (int e = 0; e < 16; ++e) {                                                \
+    const int r = wr * 64 + h * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);                    \
+    const int c = wc * 64 + mj * 32 + (lane & 31);                                                \
+    Ct[r * CS + c] = acc[2 * (PH) + h][mj][e];                                                    \
+  }                                                                                               \
+  __syncthreads();                                                                                \
+  epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, PSCALE);                                \
+  epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, PSCALE);                \
+  __syncthreads();
+  PSDP_STAGE8(0)
+  PSDP_STAGE8(1)
+#undef PSDP_STAGE8
+}
+
+template <int FMT>
+__global__ __launch_bounds__(512) void sym_prod_dma8_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, 8, false>::LDS_BYTES];
+  const TileRef t = tiles[blockIdx.x];
+  if (t.bi < 0) return;
+  sym_tile_dma8<FMT>(d, t.bi, t.bj, smem);
+}
+
 template <int FMT, int SUB>
 __global__ __launch_bounds__(256) void sym_prod_dma_grouped_kernel(const ProdDesc* __restrict__ descs,
                                                                    const TileRef* __restrict__ tiles, int ntiles) {
@@ -831,8 +994,18 @@ bool sym_big_tile_capable(int fmt, int avg) {
 // Producer waves (WS) are kept available but off: measured at n = 16384 they
 // ran the 128 x 128 FP16x3 tiles 5% slower than the plain kernel, whose
 // 64-element slabs do not fit the 256 registers a second wave per SIMD leaves.
+// 256 x 256 tiles of the two-piece formats run on 8 waves (sym_tile_dma8):
+// measured at n = 16384 (alternating runs on one device) a binary16x3
+// product takes 11.31-11.40 ms against 12.10-12.15 ms with 4 waves.  One-piece
+// formats keep 4 waves: with half the MFMAs per slab the 8-wave layout's
+// fragment reads (24 KB per wave per slab) make LDS the limit (binary16:
+// 4.31 ms against 4.23 ms).
 template <int FMT, int SUB>
 static hipError_t launch_dma(const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
+  if constexpr (SUB == 8 && FmtTraits<FMT>::P == 2) {
+    hipLaunchKernelGGL((sym_prod_dma8_kernel<FMT>), dim3(ntiles), dim3(512), 0, s, d, tiles);
+    return hipGetLastError();
+  }
   constexpr bool WS = false;
   hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, SUB, WS>), dim3(ntiles), dim3(WS ? 512 : 256), 0, s, d, tiles);
   return hipGetLastError();
