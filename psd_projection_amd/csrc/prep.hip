@@ -29,6 +29,34 @@ __device__ __forceinline__ void zero_row_tail(float* __restrict__ M, int64_t ld,
   }
 }
 
+// s1[c][r] = X(i0 + r, j0 + c), s2[c][r] = X(j0 + r, i0 + c) for the 64 x 64
+// tile pair (zero outside the n x n matrix; X column-major, leading dimension
+// xld).  Every load reads a valid address (entry 0 stands in for those
+// outside the matrix, n >= 1), so a thread issues all 32 of its loads before
+// one wait; with the bound check on the load itself each one was a branch
+// followed by its own wait.
+template <typename TIn, typename TS>
+__device__ __forceinline__ void load_tile_pair(const TIn* X, int64_t xld, int n, int i0, int j0,
+                                               TS (*s1)[65], TS (*s2)[65]) {
+  const int t = threadIdx.x, r = t & 63;
+  const auto Xg = gbl(X);
+  TIn v1[16], v2[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = (t >> 6) + 4 * k;
+    const int gi = i0 + r, gj = j0 + c, hi = j0 + r, hj = i0 + c;
+    v1[k] = Xg[(gi < n && gj < n) ? (int64_t)gi + (int64_t)gj * xld : 0];
+    v2[k] = Xg[(hi < n && hj < n) ? (int64_t)hi + (int64_t)hj * xld : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = (t >> 6) + 4 * k;
+    const int gi = i0 + r, gj = j0 + c, hi = j0 + r, hj = i0 + c;
+    s1[c][r] = (gi < n && gj < n) ? (TS)v1[k] : (TS)0;
+    s2[c][r] = (hi < n && hj < n) ? (TS)v2[k] : (TS)0;
+  }
+}
+
 template <typename TIn>
 __global__ __launch_bounds__(256) void sym_pad_kernel(const TIn* __restrict__ X, int64_t xld, int n,
                                                       float* __restrict__ M, int64_t ld) {
@@ -39,14 +67,7 @@ __global__ __launch_bounds__(256) void sym_pad_kernel(const TIn* __restrict__ X,
   const int t = threadIdx.x;
   const int r = t & 63;
   const int i0 = bi * 64, j0 = bj * 64;
-  // s1[c][r] = X(i0 + r, j0 + c); s2[c][r] = X(j0 + r, i0 + c)
-#pragma unroll 4
-  for (int c = t >> 6; c < 64; c += 4) {
-    const int gi = i0 + r, gj = j0 + c;
-    s1[c][r] = (gi < n && gj < n) ? (float)X[(int64_t)gi + (int64_t)gj * xld] : 0.0f;
-    const int hi = j0 + r, hj = i0 + c;
-    s2[c][r] = (hi < n && hj < n) ? (float)X[(int64_t)hi + (int64_t)hj * xld] : 0.0f;
-  }
+  load_tile_pair<TIn, float>(X, xld, n, i0, j0, s1, s2);
   __syncthreads();
   // M[i0 + a][j0 + b] = (X(i0+a, j0+b) + X(j0+b, i0+a)) / 2 = (s1[b][a] + s2[a][b]) / 2
 #pragma unroll 4
@@ -70,13 +91,7 @@ __global__ __launch_bounds__(256) void sym_pad_kernel<double>(const double* __re
   const int t = threadIdx.x;
   const int r = t & 63;
   const int i0 = bi * 64, j0 = bj * 64;
-#pragma unroll 4
-  for (int c = t >> 6; c < 64; c += 4) {
-    const int gi = i0 + r, gj = j0 + c;
-    s1[c][r] = (gi < n && gj < n) ? X[(int64_t)gi + (int64_t)gj * xld] : 0.0;
-    const int hi = j0 + r, hj = i0 + c;
-    s2[c][r] = (hi < n && hj < n) ? X[(int64_t)hi + (int64_t)hj * xld] : 0.0;
-  }
+  load_tile_pair<double, double>(X, xld, n, i0, j0, s1, s2);
   __syncthreads();
 #pragma unroll 4
   for (int a = t >> 6; a < 64; a += 4) {
@@ -123,15 +138,8 @@ __global__ __launch_bounds__(256) void sym_pad_batched_kernel(const PadJob* __re
   const int i0 = bi * 64, j0 = bj * 64;
   const int n = J.n;
   const int64_t ld = J.npad;
-  const auto Xg = gbl(J.X);
   const auto Mg = gbl(J.M);
-#pragma unroll 4
-  for (int c = t >> 6; c < 64; c += 4) {
-    const int gi = i0 + r, gj = j0 + c;
-    s1[c][r] = (gi < n && gj < n) ? Xg[(int64_t)gi + (int64_t)gj * n] : 0.0f;
-    const int hi = j0 + r, hj = i0 + c;
-    s2[c][r] = (hi < n && hj < n) ? Xg[(int64_t)hi + (int64_t)hj * n] : 0.0f;
-  }
+  load_tile_pair<float, float>(J.X, n, n, i0, j0, s1, s2);
   __syncthreads();
 #pragma unroll 4
   for (int a = t >> 6; a < 64; a += 4) {
