@@ -306,7 +306,7 @@ def bench_served(args, P, W, torch):
         ns = rng.choice(sizes, size=count, p=weights)
         return [Request(i, int(n), pool[int(n)][i % 4][0], 0.0) for i, n in enumerate(ns)]
 
-    ex = TorchExecutor(0, args.method, small_lanes=2, large_lanes=2, seed=5)
+    ex = TorchExecutor(0, args.method, small_lanes=1, large_lanes=2, seed=5)
     ex.warm(sizes)
     budget = 64 << 30
     # capacity: a burst (everything arrives at once)
@@ -334,7 +334,8 @@ def bench_served(args, P, W, torch):
     st = latency_stats(done)
     e = torch.stack(errs).cpu().numpy() if errs else np.zeros(1)
     res = {"workload": f"{len(reqs)} requests, Poisson at {rate:.1f}/s = {args.serve_load} x "
-                       f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}",
+                       f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}; "
+                       f"lanes: 1 small (grouped batches of n <= 1024) + 2 large",
            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
            "proj_per_s": round(st["proj_per_s"], 1), "capacity_proj_per_s": round(cap, 1),
            "launches": srv.batches, "peak_admitted_bytes": srv.peak_used,

@@ -49,7 +49,12 @@ class Request:
 class TorchExecutor:
     """Runs lanes on HIP streams through libpsdp (GPU)."""
 
-    def __init__(self, device=0, method="mixed", small_lanes=2, large_lanes=2, seed=5):
+    # One small lane: whatever arrives while a batch runs forms the next batch.
+    # Measured on the 10,000-request stream at 0.7 x capacity: 1 small + 2
+    # large lanes gave p50 9.2 ms / p99 110 ms (small requests' p99 34 ms),
+    # 2 + 2 gave p50 20 ms / p99 109 ms (small requests' p99 77 ms) at 3.5%
+    # more capacity.
+    def __init__(self, device=0, method="mixed", small_lanes=1, large_lanes=2, seed=5):
         import torch
         from . import Handle, default_config, workspace_size, workspace_size_batched
         self.torch = torch
