@@ -58,11 +58,13 @@ def test_sym_product(handle, fmt, avg, n):
     assert err <= 2e-6, err
 
 
-@pytest.mark.parametrize("fmt", ["f16", "f16x3"])
+@pytest.mark.parametrize("fmt", ["f16", "f16x3", "bf16", "bf16x3"])
 def test_sym_product_split_tail(handle, fmt):
     # n = 6144: 300 tiles of 256 x 256, i.e. one round on 256 CUs plus a partial
     # one that runs as 64 x 64 tiles in a second launch (on devices with
-    # fewer CUs the split differs; the result must not).
+    # fewer CUs the split differs; the result must not).  The two-piece
+    # formats run their 256 x 256 tiles on the 8-wave kernel, the one-piece
+    # formats on the 4-wave kernel.
     # The built-in choice at this size is 64 x 64 tiles; 256 x 256 is set by
     # hand, and both must give the same bits.
     n = 6144
