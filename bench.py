@@ -438,7 +438,9 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_TFLOPS[fmt],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
                 "traffic": measured_traffic(n, fmt),
-                "kernel": f"symmetric product, {fmt} operands (sym_prod_dma_kernel, autotuned tiles "
+                "kernel": f"symmetric product, {fmt} operands ("
+                          f"{'sym_prod_dma8_kernel, 8 waves' if fmt in ('f16x3', 'bf16x3') else 'sym_prod_dma_kernel, 4 waves'}"
+                          f" on 256x256 tiles; autotuned tiles "
                           f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds + the last partial "
                           "round as 64x64 tiles in a second launch)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
