@@ -19,15 +19,16 @@ def _inputs(sizes, seed0=700):
     return xs, ps
 
 
+@pytest.mark.parametrize("small_lanes", [1, 2])
 @pytest.mark.parametrize("staggered", [False, True])
-def test_served_results_equal_alone_results(staggered):
+def test_served_results_equal_alone_results(staggered, small_lanes):
     sizes = [64, 256, 512, 64, 1100, 256, 2100, 128, 1024, 64]
     xs, ps = _inputs(sizes)
-    ex = TorchExecutor(0, "mixed", small_lanes=2, large_lanes=2, seed=5)
+    ex = TorchExecutor(0, "mixed", small_lanes=small_lanes, large_lanes=2, seed=5)
     srv = ProjectionServer(ex, hbm_budget=8 << 30, small_max=1024)
     reqs = [Request(i, n, x, 0.0) for i, (n, x) in enumerate(zip(sizes, xs))]
     arrivals = [0.002 * i for i in range(len(reqs))] if staggered else [0.0] * len(reqs)
-    print(f"served mix sizes={sizes} staggered={staggered}", flush=True)
+    print(f"served mix sizes={sizes} staggered={staggered} small_lanes={small_lanes}", flush=True)
     done = srv.run(reqs, arrivals)
     torch.cuda.synchronize()
     assert len(done) == len(reqs) and srv.used == 0
