@@ -22,16 +22,29 @@ namespace detail {
 inline void report(psdp_status_t s, const char* what) {
   if (s != PSDP_OK) std::fprintf(stderr, "psdp error %s in %s: %s\n", psdp_status_string(s), what, psdp_last_error());
 }
-// The reference sizes workspaces as 3 n^2 floats; pass such a buffer on only
-// when it is large enough for this library's layout, else use the handle's
-// arena.
+// The reference sizes workspaces as 3 n^2 floats; this library's layout
+// (padded rows, operand pieces, Lanczos vectors) needs psdp_workspace_size
+// bytes.  A caller buffer that large is used; a smaller one (a reference-
+// sized buffer always is) is not, and the call takes the handle's arena
+// instead, which allocates device memory on first use: that is said once
+// per process on stderr, so a caller who sized the buffer to avoid
+// allocation learns that it did not.
 inline size_t usable_ws(int method, int64_t n, const void* ws, size_t ws_bytes) {
   if (!ws) return 0;
   psdp_config_t cfg;
   psdp_default_config(&cfg, method);
   size_t need = 0;
   if (psdp_workspace_size(&cfg, n, &need) != PSDP_OK) return 0;
-  return ws_bytes >= need ? ws_bytes : 0;
+  if (ws_bytes >= need) return ws_bytes;
+  static bool told = false;
+  if (!told) {
+    told = true;
+    std::fprintf(stderr,
+                 "psd_projection: workspace of %zu bytes is below the %zu bytes this library needs at n = %lld "
+                 "(psdp_workspace_size); using the handle's arena instead\n",
+                 ws_bytes, need, (long long)n);
+  }
+  return 0;
 }
 }  // namespace detail
 
@@ -92,18 +105,26 @@ inline void approximate_two_norm(psdp_handle_t h, psdp_handle_t /*cusolverH*/, c
                  "approximate_two_norm");
 }
 
-// eig_FP64_psd.h:14-20.  With return_eigenvalues the reference returns a
-// device array the caller frees; here the caller passes that array (n
-// doubles) or nullptr, and nothing uninitialised is ever returned.
-inline void eig_FP64_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
-                         double* eigenvalues_out = nullptr) {
-  detail::report(psdp_eig_fp64_psd(h, mat, (int64_t)mat_size, eigenvalues_out), "eig_FP64_psd");
+// eig_FP64_psd.h:14-20.  As in the reference, with return_eigenvalues the
+// eigenvalues (ascending, before clamping) come back in a device array of n
+// doubles that the caller frees (hipFree or psdp_free); without it the
+// result is nullptr (the reference returns an uninitialised pointer there,
+// src/eig_FP64_psd.cu:30,84).
+inline double* eig_FP64_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
+                            bool return_eigenvalues = false) {
+  double* w = nullptr;
+  detail::report(psdp_eig_fp64_psd_alloc(h, mat, (int64_t)mat_size, return_eigenvalues ? &w : nullptr),
+                 "eig_FP64_psd");
+  return w;
 }
 
 // eig_FP32_psd.h:14-20
-inline void eig_FP32_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
-                         double* eigenvalues_out = nullptr) {
-  detail::report(psdp_eig_fp32_psd(h, mat, (int64_t)mat_size, eigenvalues_out), "eig_FP32_psd");
+inline double* eig_FP32_psd(psdp_handle_t h, psdp_handle_t /*cublasH*/, double* mat, size_t mat_size,
+                            bool return_eigenvalues = false) {
+  double* w = nullptr;
+  detail::report(psdp_eig_fp32_psd_alloc(h, mat, (int64_t)mat_size, return_eigenvalues ? &w : nullptr),
+                 "eig_FP32_psd");
+  return w;
 }
 
 // composite_FP32.h:36-45.  verbose: diagnostics come back through the

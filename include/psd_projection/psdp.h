@@ -88,6 +88,15 @@ typedef enum psdp_method {
 
 #define PSDP_MAX_STEPS 16
 
+/* Residual of the Lanczos bound up = sqrt(theta + ||r||) (theta: largest
+ * Ritz value of A^T A, y: its unit Ritz vector). */
+typedef enum psdp_bound_mode {
+  PSDP_BOUND_REFERENCE = 0, /* r = A y - theta y, as src/lanczos.cu:255 computes it (default) */
+  PSDP_BOUND_TIGHT = 1      /* r = A (A y) - theta y: the Ritz pair's own residual (one more   */
+                            /* matrix-vector product); up then exceeds ||A||_2 by the Ritz     */
+                            /* error only, so clustered spectra keep more of [-1, 1]            */
+} psdp_bound_mode_t;
+
 typedef struct psdp_config {
   int method;                     /* psdp_method_t                                   */
   int table;                      /* psdp_table_t (CUSTOM only)                      */
@@ -99,12 +108,20 @@ typedef struct psdp_config {
   double scale_margin;            /* s = up * margin; default 1.0 (reference)        */
   int synchronize;                /* 1: wait for the stream and check device flags   */
   int average_products;           /* CUSTOM: 1-piece 16-bit steps use (C + C^T)/2    */
+  int bound_mode;                 /* psdp_bound_mode_t (default REFERENCE)           */
 } psdp_config_t;
 
 typedef struct psdp_handle_st* psdp_handle_t;
 
 /* Handle: device, stream (default: a new non-blocking stream owned by the
- * handle), workspace arena, Lanczos state, cached tile lists. */
+ * handle), workspace arena, Lanczos state, cached tile lists.
+ * psdp_set_stream: later calls enqueue on hip_stream, which first waits (on
+ * the device) for the work already enqueued through the handle, because that
+ * work and the next call share the handle's workspace.  One handle serves one
+ * host thread at a time; use one handle per concurrent stream of work.
+ * Calls with cfg->synchronize != 0 (the default, and every reference drop-in)
+ * return after the result is written; with synchronize == 0 they return once
+ * the work is enqueued (psdp_synchronize waits for it). */
 psdp_status_t psdp_create(psdp_handle_t* handle, int device);
 psdp_status_t psdp_destroy(psdp_handle_t handle);
 psdp_status_t psdp_set_stream(psdp_handle_t handle, void* hip_stream);
@@ -121,8 +138,17 @@ psdp_status_t psdp_workspace_size(const psdp_config_t* cfg, int64_t n, size_t* b
 
 /* Lanczos bound of the last auto-scaled projection (waits for the stream). */
 psdp_status_t psdp_last_bound(psdp_handle_t handle, double* lo, double* up, int* iterations);
+/* Per-block bounds of the handle's last batched (or packed) call: lo, up
+ * (host arrays of count doubles, either may be NULL) and iterations (host,
+ * count ints, may be NULL); count must equal that call's block count.  Waits
+ * for the stream.  Valid until the next call that uses the handle's arena. */
+psdp_status_t psdp_last_bounds_batched(psdp_handle_t handle, double* lo, double* up, int* iterations, int count);
 
-/* ---- drop-ins of the reference entry points (in place, FP64, synchronous) ---- */
+/* ---- drop-ins of the reference entry points (in place, FP64) ----
+ * Like the reference, these return after the result is written: they wait
+ * for the handle's stream and report a non-finite input, bound or result
+ * (an unscaled spectrum outside [-1, 1] makes the chain diverge,
+ * include/psd_projection/composite_FP32.h:11) as PSDP_ERR_NONFINITE. */
 psdp_status_t psdp_composite_fp32(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes);
 psdp_status_t psdp_composite_fp32_auto_scale(psdp_handle_t h, double* mat, int64_t n, void* ws,
                                              size_t ws_bytes);
@@ -217,6 +243,27 @@ psdp_status_t psdp_workspace_size_batched(const psdp_block_t* blocks, int count,
  * receives the eigenvalues of the input (ascending) before clamping. */
 psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
 psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues);
+/* The reference's ownership: with eigenvalues_out != NULL the library
+ * allocates n doubles of device memory (hipMalloc), writes the eigenvalues
+ * there and stores the pointer in *eigenvalues_out; the caller frees it with
+ * hipFree or psdp_free (src/eig_FP64_psd.cu:46-49,84; the MEX caller frees it,
+ * MATLAB/psd_projection_MATLAB.cu:158-162,175-179).  With eigenvalues_out ==
+ * NULL nothing is allocated.  On failure *eigenvalues_out is NULL. */
+psdp_status_t psdp_eig_fp64_psd_alloc(psdp_handle_t h, double* mat, int64_t n, double** eigenvalues_out);
+psdp_status_t psdp_eig_fp32_psd_alloc(psdp_handle_t h, double* mat, int64_t n, double** eigenvalues_out);
+/* Frees device memory the library allocated for the caller (NULL: no-op). */
+psdp_status_t psdp_free(void* device_ptr);
+
+/* ---- method-string front end (the MEX gateway's dispatch,
+ * MATLAB/psd_projection_MATLAB.cu:112-166, method set README.md:56) ----
+ * In-place float64 projection of mat by name: "composite_FP32",
+ * "composite_FP16", "composite_FP32_emulated" (the *_auto_scale calls),
+ * "eig_FP64", "eig_FP32" (EVD; eigenvalues: NULL or n device doubles that
+ * receive the input's eigenvalues, as the MEX call returns them), or
+ * "mixed" (the shipped schedule, auto-scaled).  Unknown names return
+ * PSDP_ERR_INVALID_ARG.  Waits for the result, as the MEX call does. */
+psdp_status_t psdp_project_method(psdp_handle_t h, const char* method, double* mat, int64_t n,
+                                  double* eigenvalues);
 
 /* ---- extremal eigenpairs and deflated projections (float64, device arrays) ---- */
 

@@ -45,6 +45,12 @@ METHOD_FP32, METHOD_FP16, METHOD_FP32_EMULATED, METHOD_MIXED, METHOD_CUSTOM = ra
 METHODS = {"composite_FP32": METHOD_FP32, "composite_FP16": METHOD_FP16,
            "composite_FP32_emulated": METHOD_FP32_EMULATED, "mixed": METHOD_MIXED,
            "custom": METHOD_CUSTOM}
+# The MEX gateway's method strings (MATLAB/psd_projection_MATLAB.cu:112-166,
+# README.md:56) plus the shipped schedule; psdp_project_method dispatches them.
+MEX_METHODS = ("composite_FP16", "composite_FP32", "composite_FP32_emulated", "eig_FP64", "eig_FP32", "mixed")
+EVD_METHODS = ("eig_FP64", "eig_FP32")
+# psdp_bound_mode_t
+BOUND_REFERENCE, BOUND_TIGHT = 0, 1
 MAX_STEPS = 16
 # psdp_prof_kind_t
 PROF_KINDS = {0: "prod_f32", 1: "prod_f16", 2: "prod_f16x3", 3: "prod_bf16", 4: "prod_bf16x3",
@@ -77,6 +83,7 @@ class Config(ctypes.Structure):
         ("scale_margin", ctypes.c_double),
         ("synchronize", ctypes.c_int),
         ("average_products", ctypes.c_int),
+        ("bound_mode", ctypes.c_int),
     ]
 
 
@@ -101,6 +108,16 @@ _PROTOS = {
                                            ctypes.POINTER(ctypes.c_size_t)]),
     "psdp_last_bound": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "psdp_last_bounds_batched": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                                ctypes.c_int]),
+    "psdp_eig_fp64_psd_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "psdp_eig_fp32_psd_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "psdp_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "psdp_project_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64,
+                                           ctypes.c_void_p]),
     "psdp_approximate_two_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                  ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
@@ -188,6 +205,8 @@ def default_config(method="mixed", **overrides) -> Config:
             cfg.final_format = FORMATS[v] if isinstance(v, str) else int(v)
         elif k == "table":
             cfg.table = {"fp32": TABLE_FP32, "fp16": TABLE_FP16}.get(v, v)
+        elif k == "bound_mode":
+            cfg.bound_mode = {"reference": BOUND_REFERENCE, "tight": BOUND_TIGHT}.get(v, v)
         else:
             setattr(cfg, k, v)
     return cfg
@@ -253,6 +272,14 @@ class Handle:
         lo, up, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _check(lib().psdp_last_bound(self._h, ctypes.byref(lo), ctypes.byref(up), ctypes.byref(it)))
         return lo.value, up.value, it.value
+
+    def last_bounds_batched(self, count):
+        """[(lo, up, iterations)] of each block of the last batched call."""
+        lo = (ctypes.c_double * count)()
+        up = (ctypes.c_double * count)()
+        it = (ctypes.c_int * count)()
+        _check(lib().psdp_last_bounds_batched(self._h, lo, up, it, int(count)))
+        return [(lo[i], up[i], it[i]) for i in range(count)]
 
     def set_eigenpair_options(self, residual_row="first", normalize=False):
         """compute_eigenpairs options: residual row "first" (the reference's)
@@ -419,6 +446,41 @@ def eig_FP32_psd(handle, mat, n=None, return_eigenvalues=False):
     return w
 
 
+def eig_psd_alloc(handle, mat, n=None, fp64=True, return_eigenvalues=True):
+    """The reference's ownership of eig_FP64_psd / eig_FP32_psd: returns the
+    device address of n eigenvalues the library allocated (the caller frees
+    it with free_device), or 0 when not requested."""
+    handle._sync_stream()
+    n = _size(mat, n)
+    out = ctypes.c_void_p()
+    fn = lib().psdp_eig_fp64_psd_alloc if fp64 else lib().psdp_eig_fp32_psd_alloc
+    _check(fn(handle.ptr, _ptr(mat), n, ctypes.byref(out) if return_eigenvalues else None))
+    return out.value or 0
+
+
+def free_device(ptr):
+    """psdp_free: releases device memory the library allocated for the caller."""
+    _check(lib().psdp_free(ctypes.c_void_p(ptr) if ptr else None))
+
+
+def project_method(handle, mat, method, n=None, return_eigenvalues=False):
+    """The MEX gateway's dispatch (MATLAB/psd_projection_MATLAB.cu:112-166):
+    projects the float64 matrix in place by method name (composite_FP16,
+    composite_FP32, composite_FP32_emulated, eig_FP64, eig_FP32, mixed).
+    Returns (mat, eigenvalues) -- eigenvalues only for eig_* with
+    return_eigenvalues, else None."""
+    import torch
+    handle._sync_stream()
+    n = _size(mat, n)
+    if mat.dtype != torch.float64:
+        raise TypeError("expected a float64 matrix")
+    w = None
+    if return_eigenvalues and method in EVD_METHODS:
+        w = torch.empty(n, dtype=torch.float64, device=mat.device)
+    _check(lib().psdp_project_method(handle.ptr, str(method).encode(), _ptr(mat), n, _ptr(w)))
+    return mat, w
+
+
 def _f64_square(A, n):
     import torch
     n = _size(A, n)
@@ -505,6 +567,12 @@ def project(handle, X, out=None, method="mixed", config: Config | None = None, w
     n = _size(X, None)
     if out is None:
         out = torch.empty_like(X)
+    if config is None and method in EVD_METHODS:
+        # the EVD methods of the MEX set: float64 in place on a copy
+        tmp = X.to(torch.float64, copy=True)
+        project_method(handle, tmp, method)
+        out.copy_(tmp)
+        return out
     cfg = config if config is not None else default_config(method, **overrides)
     ws, wsb = _ws(workspace)
     if X.dtype == torch.float64:
@@ -597,5 +665,5 @@ __all__ = [
     "composite_FP32_emulated", "composite_FP32_emulated_auto_scale", "approximate_two_norm",
     "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "project_packed", "sym_product", "workspace_size",
     "compute_eigenpairs", "lobpcg", "composite_FP32_auto_scale_deflate", "composite_FP16_auto_scale_deflate",
-    "eig_FP64_deflate",
+    "eig_FP64_deflate", "project_method", "eig_psd_alloc", "free_device",
 ]

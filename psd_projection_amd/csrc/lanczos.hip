@@ -255,7 +255,8 @@ __global__ __launch_bounds__(256) void symv_reduce_kernel(const double* __restri
   symv_reduce_body(P, m, y, (int)blockIdx.x);
 }
 
-// which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y (skip unless Ritz pair).
+// which: 0 -> w1 = A q, 1 -> w = A w1, 2 -> ay = A y, 3 -> w1 = A ay (2 and 3
+// skip unless there is a Ritz pair; 3 only for the tight bound).
 // Flat work lists over every matrix: tiles[g] = {matrix, I, J}, chunks[g] =
 // {matrix, row chunk}, so no workgroup idles on a small matrix.
 __global__ __launch_bounds__(256) void symv_tile_batched(const LanczosJob* __restrict__ jobs,
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256) void symv_tile_batched(const LanczosJob* __res
   const TileRef t = tiles[blockIdx.x];
   const LanczosJob& J = jobs[t.mat];
   if (which < 2 ? J.st->done : J.st->skip_ritz) return;
-  const double* x = which == 0 ? J.v.q : (which == 1 ? J.v.w1 : J.v.y);
+  const double* x = which == 0 ? J.v.q : (which == 1 ? J.v.w1 : (which == 2 ? J.v.y : J.v.ay));
   symv_tile_body(J.A, J.ld, J.m, x, J.v.part, t.bi, t.bj);
 }
 
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(256) void symv_reduce_batched(const LanczosJob* __r
   const int2 ch = chunks[blockIdx.x];
   const LanczosJob& J = jobs[ch.x];
   if (which < 2 ? J.st->done : J.st->skip_ritz) return;
-  double* y = which == 0 ? J.v.w1 : (which == 1 ? J.v.w : J.v.ay);
+  double* y = which == 0 ? J.v.w1 : (which == 1 ? J.v.w : (which == 2 ? J.v.ay : J.v.w1));
   symv_reduce_body(J.v.part, J.m, y, ch.y);
 }
 
@@ -305,6 +306,7 @@ __device__ void init_body(LanczosState* st, const LanczosVecs& v, int n, int m, 
     st->done = 0;
     st->skip_ritz = 0;
     st->nonfinite = 0;
+    st->out_nonfinite = 0;
     st->theta = 0.0;
     st->rnorm = 0.0;
     st->lo = 0.0;
@@ -511,13 +513,17 @@ __device__ __forceinline__ void ritz_vector_body(const LanczosState* st, const L
 
 // r = A y - theta y; lo = sqrt(theta); up = sqrt(theta + ||r||)
 // (src/lanczos.cu:258-270); s = up > 0 ? up : 1 (src/composite_FP32.cu:182).
-__device__ void bound_body(LanczosState* st, const LanczosVecs& v, int m, double margin) {
+// tight: r = A (A y) - theta y (v.w1 holds A A y), the residual of the Ritz
+// pair of A^T A itself, so that up = sqrt(theta + ||r||) exceeds ||A||_2 by
+// the Ritz pair's own error instead of by |lambda - lambda^2|.
+__device__ void bound_body(LanczosState* st, const LanczosVecs& v, int m, double margin, int tight) {
   __shared__ double red[16];
   if (st->skip_ritz) return;
   const double th = st->theta;
+  const double* ry = tight ? v.w1 : v.ay;
   double s = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double r = fma(-th, v.y[i], v.ay[i]);
+    const double r = fma(-th, v.y[i], ry[i]);
     s += r * r;
   }
   const double rn = sqrt(block_sum(s, red));
@@ -549,8 +555,8 @@ __global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st
   ritz_vector_body(st, v, m, blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, LanczosVecs v, int m,
-                                                             double margin) {
-  bound_body(st, v, m, margin);
+                                                             double margin, int tight) {
+  bound_body(st, v, m, margin, tight);
 }
 
 // ------------------------------------------------------------------ batched
@@ -567,9 +573,9 @@ __global__ __launch_bounds__(256) void ritz_vector_batched(const LanczosJob* job
   const LanczosJob& J = jobs[blockIdx.y];
   ritz_vector_body(J.st, J.v, J.m, blockIdx.x * blockDim.x + threadIdx.x);
 }
-__global__ __launch_bounds__(1024) void lanczos_bound_batched(const LanczosJob* jobs, double margin) {
+__global__ __launch_bounds__(1024) void lanczos_bound_batched(const LanczosJob* jobs, double margin, int tight) {
   const LanczosJob& J = jobs[blockIdx.x];
-  bound_body(J.st, J.v, J.m, margin);
+  bound_body(J.st, J.v, J.m, margin, tight);
 }
 
 // Fixed scale (no bound): used by the non-auto-scaled entry points.
@@ -578,6 +584,7 @@ __global__ void set_scale_kernel(LanczosState* st, double scale) {
     st->scale = scale;
     st->inv_scale = 1.0 / scale;
     st->nonfinite = 0;
+    st->out_nonfinite = 0;
   }
 }
 
@@ -605,7 +612,7 @@ static void launch_symv(const float* A, int64_t ld, int m, const double* x, doub
 // logical size n <= m (entries beyond n are zero).  Vectors have length m.
 // A float32 A needs m % 4 == 0, ld % 4 == 0, 16-byte alignment and v.part.
 hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
-                          uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
+                          uint64_t seed, double margin, int tight, LanczosState* st, const LanczosVecs& v,
                           hipStream_t s) {
   if (!a_f64 && (m % 4 != 0 || ld % 4 != 0 || ((uintptr_t)A) % 16 != 0 || !v.part)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lanczos_init_kernel, dim3(1), dim3(1024), 0, s, st, v, n, m, seed);
@@ -625,7 +632,13 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
     launch_gemv((const double*)A, ld, m, v.y, v.ay, &st->skip_ritz, s);
   else
     launch_symv((const float*)A, ld, m, v.y, v.ay, v.part, &st->skip_ritz, s);
-  hipLaunchKernelGGL(lanczos_bound_kernel, dim3(1), dim3(1024), 0, s, st, v, m, margin);
+  if (tight) {
+    if (a_f64)
+      launch_gemv((const double*)A, ld, m, v.ay, v.w1, &st->skip_ritz, s);
+    else
+      launch_symv((const float*)A, ld, m, v.ay, v.w1, v.part, &st->skip_ritz, s);
+  }
+  hipLaunchKernelGGL(lanczos_bound_kernel, dim3(1), dim3(1024), 0, s, st, v, m, margin, tight);
   return hipGetLastError();
 }
 
@@ -635,7 +648,7 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
 // triangle tile (I <= J < ceil(m / SYMV_TILE)) of every matrix; chunks:
 // {matrix, c} for every SYMV_TILE-row chunk c of every matrix.
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, const TileRef* tiles, int ntiles,
+                                  uint64_t seed, double margin, int tight, const TileRef* tiles, int ntiles,
                                   const int2* chunks, int nchunks, hipStream_t s) {
   const dim3 tgrid(ntiles), rgrid(nchunks);
   hipLaunchKernelGGL(lanczos_init_batched, dim3(count), dim3(1024), 0, s, jobs, seed);
@@ -648,9 +661,11 @@ hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, 
   }
   hipLaunchKernelGGL(lanczos_ritz_batched, dim3(count), dim3(64), 0, s, jobs);
   hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
-  hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, 2);
-  hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, 2);
-  hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin);
+  for (int which = 2; which < (tight ? 4 : 3); ++which) {
+    hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, which);
+    hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, which);
+  }
+  hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin, tight);
   return hipGetLastError();
 }
 

@@ -38,13 +38,13 @@ hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M
 hipError_t launch_scale_split(float* a32, int64_t count, const double* inv, int fmt_a, uint16_t* const ap[3],
                               int fmt_x, float* x32, uint16_t* const xp[3], hipStream_t st);
 hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
-                          uint64_t seed, double margin, LanczosState* st, const LanczosVecs& v,
+                          uint64_t seed, double margin, int tight, LanczosState* st, const LanczosVecs& v,
                           hipStream_t s);
 hipError_t launch_set_scale(LanczosState* st, double scale, hipStream_t s);
 hipError_t launch_fill_sym(float* M, int64_t ld, int n, int npad, uint64_t seed, hipStream_t st);
 hipError_t launch_copy_words(void* dst, const void* src, int64_t words, hipStream_t st);
 hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, int max_iter, double tol,
-                                  uint64_t seed, double margin, const TileRef* tiles, int ntiles,
+                                  uint64_t seed, double margin, int tight, const TileRef* tiles, int ntiles,
                                   const int2* chunks, int nchunks, hipStream_t s);
 hipError_t launch_sym_pad_batched(const PadJob* jobs, const TileRef* tiles, int ntiles, hipStream_t st);
 hipError_t launch_scale_split_batched(const SplitJob* jobs, const int2* chunks, int nchunks, int fmt_a, int fmt_x,
@@ -217,6 +217,10 @@ struct psdp_handle_st {
   int batch_big_n = 0;  // 0: BIG_BLOCK_N
   std::map<std::tuple<int, int, int>, int> tuned;  // (n, format, averaged) -> tile sub (psdp_autotune)
   int batch_concurrent = 0;
+  // states of the last batched call (in the arena) for psdp_last_bounds_batched
+  const LanczosState* batch_states = nullptr;
+  int batch_count = 0;
+  hipEvent_t order_ev = nullptr;  // orders a new stream after work on the previous one
   hipStream_t aux = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // batched calls replayed from a captured HIP graph (psdp_set_batched_graph):
@@ -363,6 +367,8 @@ static psdp_status_t get_split_tiles(psdp_handle_t h, int T, TileList& big, Tile
 }
 
 static psdp_status_t arena_reserve(psdp_handle_t h, size_t bytes, void** out) {
+  h->batch_states = nullptr;  // the caller overwrites the arena: the last batch's bounds go
+  h->batch_count = 0;
   if (bytes > h->arena_bytes) {
     if (h->arena) {
       HIPCHK(hipStreamSynchronize(h->stream));
@@ -625,7 +631,8 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     const double margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
     const int iters = std::min(p.lanczos_iters, p.n);  // max_iter = min(max_iter, n), src/lanczos.cu:55
     ProfScope ps(h, PSDP_PROF_LANCZOS);
-    HIPCHK(launch_lanczos(b.a32, 0, p.ld, p.n, p.npad, iters, tol, h->seed, margin, h->st, b.v, s));
+    HIPCHK(launch_lanczos(b.a32, 0, p.ld, p.n, p.npad, iters, tol, h->seed, margin,
+                          cfg->bound_mode == PSDP_BOUND_TIGHT, h->st, b.v, s));
   } else {
     HIPCHK(launch_set_scale(h->st, fixed_scale, s));
   }
@@ -679,6 +686,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   d4.out_ld = p.n;
   d4.out_f64 = y_f64;
   d4.scale = &h->st->scale;
+  d4.nonfinite = &h->st->out_nonfinite;
   STCHK(product(sch.final_fmt, sch.final_avg, d4));
   return PSDP_OK;
 }
@@ -696,6 +704,10 @@ static psdp_status_t check_flags(psdp_handle_t h) {
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
   if (st.nonfinite) return fail(PSDP_ERR_NONFINITE, "non-finite value in the input or the spectral bound");
+  if (st.out_nonfinite)
+    return fail(PSDP_ERR_NONFINITE,
+                "non-finite value in the result: the filter chain diverged (a spectrum outside [-1, 1] "
+                "needs the auto-scaled entry point)");
   return PSDP_OK;
 }
 
@@ -838,6 +850,7 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   if (h->bgraph) (void)hipGraphExecDestroy(h->bgraph);
   if (h->cap) (void)hipStreamDestroy(h->cap);
   if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  if (h->order_ev) (void)hipEventDestroy(h->order_ev);
   if (h->join_ev) (void)hipEventDestroy(h->join_ev);
   if (h->aux) (void)hipStreamDestroy(h->aux);
   if (h->pinned) (void)hipHostFree(h->pinned);
@@ -848,15 +861,26 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   return PSDP_OK;
 }
 
+// Work already enqueued on the previous stream may still read or write the
+// handle's arena, Lanczos state and staging memory, which the next call
+// reuses: the new stream waits for an event recorded on the old one (no host
+// wait), so calls through one handle stay ordered across stream changes.
 psdp_status_t psdp_set_stream(psdp_handle_t h, void* stream) {
   if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
   std::lock_guard<std::mutex> lock(h->mu);
+  hipStream_t ns = (hipStream_t)stream;
+  if (ns == h->stream && !h->own_stream) return PSDP_OK;
+  HIPCHK(hipSetDevice(h->device));
   if (h->own_stream) {
-    (void)hipStreamSynchronize(h->stream);
-    (void)hipStreamDestroy(h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamDestroy(h->stream));
     h->own_stream = false;
+  } else {
+    if (!h->order_ev) HIPCHK(hipEventCreateWithFlags(&h->order_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(h->order_ev, h->stream));
+    HIPCHK(hipStreamWaitEvent(ns, h->order_ev, 0));
   }
-  h->stream = (hipStream_t)stream;
+  h->stream = ns;
   return PSDP_OK;
 }
 
@@ -1004,6 +1028,23 @@ psdp_status_t psdp_last_bound(psdp_handle_t h, double* lo, double* up, int* iter
   return PSDP_OK;
 }
 
+psdp_status_t psdp_last_bounds_batched(psdp_handle_t h, double* lo, double* up, int* iterations, int count) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  if (!h->batch_states || count != h->batch_count)
+    return fail(PSDP_ERR_INVALID_ARG, "the last batched call had %d blocks (asked for %d)", h->batch_count, count);
+  std::vector<LanczosState> st((size_t)count);
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(st.data(), h->batch_states, sizeof(LanczosState) * count, hipMemcpyDeviceToHost));
+  for (int b = 0; b < count; ++b) {
+    if (lo) lo[b] = st[b].lo;
+    if (up) up[b] = st[b].up;
+    if (iterations) iterations[b] = st[b].nb;
+  }
+  return PSDP_OK;
+}
+
 psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int64_t n, const psdp_config_t* cfg,
                                void* ws, size_t ws_bytes) {
   return project(h, X, 1, Y, 1, n, cfg, ws, ws_bytes);
@@ -1061,7 +1102,7 @@ static psdp_status_t two_norm_locked(psdp_handle_t h, const double* A, int64_t n
   lv.ay = (double*)(v + 5 * vb);
   lv.V = (double*)(v + 6 * vb);
   lv.part = nullptr;  // float64 matrix: full-read GEMV
-  HIPCHK(launch_lanczos(A, 1, n, (int)n, (int)n, iters, tol, h->seed, 1.0, h->st, lv, h->stream));
+  HIPCHK(launch_lanczos(A, 1, n, (int)n, (int)n, iters, tol, h->seed, 1.0, 0, h->st, lv, h->stream));
   LanczosState st;
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
@@ -1200,7 +1241,7 @@ static std::vector<TileRef> batched_tile_order(const std::vector<int>& blocks, i
 
 struct BatchPlan {
   Schedule sch;
-  int iters = 20;
+  int iters = 20, tight = 0;
   double tol = 1e-10, margin = 1.0;
   int count = 0, nstages = 0, max_npad = 0;
   int ntiles_big = 0, ntiles_small = 0;  // 128 x 128 tiles first, then 64 x 64
@@ -1225,6 +1266,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
   bp.tol = cfg->lanczos_tol > 0 ? cfg->lanczos_tol : 1e-10;
   bp.margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
+  bp.tight = cfg->bound_mode == PSDP_BOUND_TIGHT;
   for (int b = 0; b < count; ++b)
     if (!blocks[b].X || !blocks[b].Y || blocks[b].n <= 0 || blocks[b].n > 65536)
       return fail(PSDP_ERR_INVALID_ARG, "block %d: bad pointer or size", b);
@@ -1418,6 +1460,7 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     d4.out_ld = q.n;
     d4.out_f64 = 0;
     d4.scale = &states[b].scale;
+    d4.nonfinite = &states[b].out_nonfinite;
   }
   for (int st = 0; st < bp.nstages; ++st) {
     const int f = st >= 3 * sch.nsteps ? sch.final_fmt : sch.fmt[st / 3];
@@ -1487,6 +1530,8 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   HIPCHK(hipMemcpyAsync(ddev, stage, bp.blob.size(), hipMemcpyHostToDevice, s));
   STCHK(staging_release(h));
 
+  h->batch_states = states;
+  h->batch_count = count;
   const LanczosJob* lz_d = (const LanczosJob*)(ddev + bp.d_lz);
   const ProdDesc* pd_d = (const ProdDesc*)(ddev + bp.d_prod);
   const TileRef* tl_d = (const TileRef*)(ddev + bp.d_tiles);
@@ -1502,7 +1547,7 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
     }
     if (cfg->auto_scale) {
       ProfScope ps(h, PSDP_PROF_LANCZOS);
-      HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin,
+      HIPCHK(launch_lanczos_batched(lz_d, count, bp.max_npad, bp.iters, bp.tol, h->seed, bp.margin, bp.tight,
                                     (const TileRef*)(ddev + bp.d_symt), bp.nsymv_tiles,
                                     (const int2*)(ddev + bp.d_rows), bp.nsymv_chunks, s));
     }
@@ -1582,8 +1627,11 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
     std::vector<LanczosState> out(count);
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipMemcpy(out.data(), states, sizeof(LanczosState) * count, hipMemcpyDeviceToHost));
-    for (int b = 0; b < count; ++b)
+    for (int b = 0; b < count; ++b) {
       if (out[b].nonfinite) return fail(PSDP_ERR_NONFINITE, "block %d: non-finite value in the input", b);
+      if (out[b].out_nonfinite)
+        return fail(PSDP_ERR_NONFINITE, "block %d: non-finite value in the result (diverged chain)", b);
+    }
   }
   return PSDP_OK;
 }
@@ -1679,6 +1727,63 @@ psdp_status_t psdp_eig_fp64_psd(psdp_handle_t h, double* mat, int64_t n, double*
 
 psdp_status_t psdp_eig_fp32_psd(psdp_handle_t h, double* mat, int64_t n, double* eigenvalues) {
   return eig_dropin(h, mat, n, eigenvalues, 0);
+}
+
+// The reference returns a device array it allocated (src/eig_FP64_psd.cu:30,
+// 46-49, 84); here it is allocated only when asked for, and never returned
+// uninitialised.
+static psdp_status_t eig_alloc(psdp_handle_t h, double* mat, int64_t n, double** out, int f64) {
+  if (out) *out = nullptr;
+  if (!h || !mat) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad matrix size %lld", (long long)n);
+  double* w = nullptr;
+  if (out) {
+    HIPCHK(hipSetDevice(h->device));
+    if (hipMalloc(&w, sizeof(double) * (size_t)n) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PSDP_ERR_NOMEM, "eigenvalue array of %lld doubles", (long long)n);
+    }
+  }
+  const psdp_status_t st = eig_dropin(h, mat, n, w, f64);
+  if (st != PSDP_OK) {
+    if (w) (void)hipFree(w);
+    return st;
+  }
+  if (out) *out = w;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_eig_fp64_psd_alloc(psdp_handle_t h, double* mat, int64_t n, double** eigenvalues_out) {
+  return eig_alloc(h, mat, n, eigenvalues_out, 1);
+}
+
+psdp_status_t psdp_eig_fp32_psd_alloc(psdp_handle_t h, double* mat, int64_t n, double** eigenvalues_out) {
+  return eig_alloc(h, mat, n, eigenvalues_out, 0);
+}
+
+psdp_status_t psdp_free(void* p) {
+  if (!p) return PSDP_OK;
+  HIPCHK(hipFree(p));
+  return PSDP_OK;
+}
+
+// MATLAB/psd_projection_MATLAB.cu:112-166: the five method strings of the
+// MEX gateway (README.md:56), each an in-place float64 projection that
+// returns when done; "mixed" adds the shipped schedule.
+psdp_status_t psdp_project_method(psdp_handle_t h, const char* method, double* mat, int64_t n,
+                                  double* eigenvalues) {
+  if (!method) return fail(PSDP_ERR_INVALID_ARG, "null method name");
+  const std::string m(method);
+  if (m == "composite_FP32") return psdp_composite_fp32_auto_scale(h, mat, n, nullptr, 0);
+  if (m == "composite_FP16") return psdp_composite_fp16_auto_scale(h, mat, n, nullptr, 0);
+  if (m == "composite_FP32_emulated") return psdp_composite_fp32_emulated_auto_scale(h, mat, n, nullptr, 0);
+  if (m == "eig_FP64") return psdp_eig_fp64_psd(h, mat, n, eigenvalues);
+  if (m == "eig_FP32") return psdp_eig_fp32_psd(h, mat, n, eigenvalues);
+  if (m == "mixed") return dropin(h, mat, n, nullptr, 0, PSDP_METHOD_MIXED, 1);
+  return fail(PSDP_ERR_INVALID_ARG,
+              "unknown method '%s' (composite_FP16, composite_FP32, composite_FP32_emulated, eig_FP64, eig_FP32, "
+              "mixed)",
+              method);
 }
 
 // ---------------------------------------------------------------- eigenpairs and deflation

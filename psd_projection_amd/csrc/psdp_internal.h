@@ -49,6 +49,7 @@ struct ProdDesc {
   int64_t out_ld;
   int out_f64;       // 1: double output, 0: float output
   const double* scale;  // EPI_OUTPUT multiplier (device scalar); null -> 1
+  int* nonfinite;    // EPI_OUTPUT: set to 1 when a written value is not finite (null: unchecked)
 };
 
 // Tile of the upper triangle: matrix index + block coordinates (bi <= bj).
@@ -81,7 +82,9 @@ struct LanczosState {
   int nb;            // completed iterations (nb_iter)
   int done;          // loop finished (break rule hit)
   int skip_ritz;     // nb == 0: no Ritz pair (all-zero input)
-  int nonfinite;     // non-finite value met
+  int nonfinite;     // non-finite value met by the bound (input or bound)
+  int out_nonfinite; // non-finite value written by the output product (diverged chain)
+  int pad_;
 };
 
 // Device vectors of one Lanczos run (length m each; V is m x max_iter).

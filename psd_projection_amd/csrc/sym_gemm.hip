@@ -269,7 +269,12 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
     auto out32 = gbl((float*)d.out);
     const double s = (d.scale ? *d.scale : 1.0) * (double)cscale;
     const int64_t old = d.out_ld;
-    // (i, j) at i + j*ld: contiguous along i
+    int* const flag = d.nonfinite;
+    // (i, j) at i + j*ld: contiguous along i.  A diverged chain (spectrum
+    // outside [-1, 1] without the bound) shows up here as Inf/NaN: the
+    // first pass covers every written value once (the mirror pass repeats
+    // them), so it alone checks them and raises the caller's flag.
+    bool bad = false;
 #pragma unroll 2
     for (int q = tid; q < BT * nq; q += NT) {
       const int c = q / nq, rr = (q % nq) * 4;
@@ -279,11 +284,13 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
         const int i = row0 + rr + e;
         if (i < n && j < n && (!diag || i <= j)) {
           const double v = (double)C[(rr + e) * CS + c] * s;
+          bad |= !isfinite(v);
           if (out_f64) out64[(int64_t)i + (int64_t)j * old] = v;
           else out32[(int64_t)i + (int64_t)j * old] = (float)v;
         }
       }
     }
+    if (bad && flag) *gbl(flag) = 1;
     // (j, i) at j + i*ld: contiguous along j
 #pragma unroll 2
     for (int q = tid; q < NR * Q4; q += NT) {

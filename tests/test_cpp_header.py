@@ -20,8 +20,15 @@ int main(int argc, char**) {
     psd_projection::composite_FP32_emulated(h, m, 10);
     psd_projection::composite_FP32_emulated_auto_scale(h, h, m, 10);
     psd_projection::approximate_two_norm(h, h, m, 10, &lo, &up);
-    psd_projection::eig_FP64_psd(h, h, m, 10);
-    psd_projection::eig_FP32_psd(h, h, m, 10);
+    // the MEX gateway's call shape (MATLAB/psd_projection_MATLAB.cu:158-162,175-179):
+    // eigenvalues returned as device memory the caller frees
+    double* dW = psd_projection::eig_FP64_psd(h, h, m, 10, true);
+    psdp_free(dW);
+    double* none = psd_projection::eig_FP32_psd(h, h, m, 10);
+    psdp_free(none);
+    dW = psd_projection::eig_FP32_psd(h, h, m, (size_t)10, true);
+    psdp_free(dW);
+    psdp_project_method(h, "eig_FP64", m, 10, nullptr);
     size_t r = 0;
     psd_projection::composite_FP32_auto_scale_deflate(h, h, m, 10);
     psd_projection::composite_FP16_auto_scale_deflate(h, h, m, 10);

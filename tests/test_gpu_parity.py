@@ -377,3 +377,114 @@ def test_eig_fp32_psd(handle):
     w = P.eig_FP32_psd(handle, d, return_eigenvalues=True)
     assert O.rel_fro_error(from_dev(d), O.closed_form_projection(Q, lam)) < 2e-5
     assert np.allclose(w.cpu().numpy(), np.sort(lam), atol=1e-4)
+
+
+# ------------------------------------------------------------------ diverged chain
+@pytest.mark.parametrize("fn", [P.composite_FP32, P.composite_FP16, P.composite_FP32_emulated])
+def test_unscaled_divergence_reported(handle, fn):
+    # include/psd_projection/composite_FP32.h:11: without the bound a spectrum
+    # outside [-1, 1] makes the chain diverge (p_1(3) ~ 3e3, then overflow);
+    # the output epilogue sees the non-finite values and the call says so.
+    d = to_dev(np.diag([3.0, -2.0]))
+    with pytest.raises(P.PsdpError) as e:
+        fn(handle, d)
+    assert e.value.status == P.ERR_NONFINITE
+    # the same matrix through the auto-scaled entry point is fine
+    d = to_dev(np.diag([3.0, -2.0]))
+    P.composite_FP32_auto_scale(handle, d)
+    assert np.abs(from_dev(d) - np.diag([3.0, 0.0])).max() < 1e-4
+
+
+def test_unscaled_nan_input_reported(handle):
+    X = torch.eye(40, dtype=torch.float64, device=DEV) * 0.5
+    X[3, 5] = X[5, 3] = float("nan")
+    with pytest.raises(P.PsdpError) as e:
+        P.composite_FP32(handle, X)
+    assert e.value.status == P.ERR_NONFINITE
+    # a later good call on the same handle is not affected by the old flag
+    Y = torch.eye(40, dtype=torch.float64, device=DEV) * 0.5
+    P.composite_FP32(handle, Y)
+    assert torch.allclose(Y, torch.eye(40, dtype=torch.float64, device=DEV) * 0.5, atol=1e-6)
+
+
+# ------------------------------------------------------------------ EVD drop-ins, reference ownership
+def _read_device_doubles(ptr, n):
+    """Copies n doubles at a raw device address to the host (hipMemcpy)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(n, dtype=np.float64)
+    st = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(8 * n), 2)
+    assert st == 0, st  # hipMemcpyDeviceToHost
+    return out
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_eig_psd_returns_owned_eigenvalues(handle, fp64):
+    # src/eig_FP64_psd.cu:46-49,84: with return_eigenvalues the call returns a
+    # device array the caller frees (MATLAB/psd_projection_MATLAB.cu:158-162,
+    # 175-179); without it nothing is allocated (null, never uninitialised).
+    X, Q, lam = O.spectral_input(150, 23, "normal")
+    d = to_dev(X)
+    ptr = P.eig_psd_alloc(handle, d, fp64=fp64, return_eigenvalues=True)
+    assert ptr != 0
+    w = _read_device_doubles(ptr, 150)
+    P.free_device(ptr)
+    tol = 1e-12 if fp64 else 1e-4
+    assert np.allclose(w, np.sort(lam), atol=tol)
+    assert O.rel_fro_error(from_dev(d), O.closed_form_projection(Q, lam)) < (1e-12 if fp64 else 2e-5)
+    d2 = to_dev(X)
+    assert P.eig_psd_alloc(handle, d2, fp64=fp64, return_eigenvalues=False) == 0
+    assert torch.equal(d, d2)
+
+
+# ------------------------------------------------------------------ method strings (MEX dispatch)
+@pytest.mark.parametrize("method,bar", [("composite_FP32", 2e-5), ("composite_FP16", 5e-3),
+                                        ("composite_FP32_emulated", 2e-5), ("eig_FP64", 1e-12),
+                                        ("eig_FP32", 2e-5), ("mixed", 1e-3)])
+def test_method_strings(handle, method, bar):
+    # MATLAB/psd_projection_MATLAB.cu:112-166: one entry point keyed by the
+    # method string, in place on a float64 matrix; the eig_* methods also
+    # return the eigenvalues.
+    X, Q, lam = O.spectral_input(400, 24, "uniform")
+    Pt = O.closed_form_projection(Q, lam)
+    d = to_dev(X)
+    _, w = P.project_method(handle, d, method, return_eigenvalues=True)
+    err = O.rel_fro_error(from_dev(d), Pt)
+    print(f"method {method}: rel err {err:.3e}", flush=True)
+    assert err <= bar
+    if method.startswith("eig_"):
+        assert np.allclose(w.cpu().numpy(), np.sort(lam), atol=1e-4)
+    else:
+        assert w is None
+    # the float32 front end takes the same names
+    Y = P.project(handle, to_dev(X, torch.float32), method=method)
+    assert O.rel_fro_error(from_dev(Y), Pt) <= max(bar, 2e-5)
+    with pytest.raises(P.PsdpError) as e:
+        P.project_method(handle, d, "eig_FP16")
+    assert e.value.status == P.ERR_INVALID_ARG
+
+
+# ------------------------------------------------------------------ stream changes
+def test_two_streams_one_handle_ordered(handle):
+    # Two asynchronous projections through one handle on two torch streams:
+    # they share the handle's workspace, so the second stream must wait for the
+    # first one's work (psdp_set_stream orders them); both results equal the
+    # synchronous ones bitwise.
+    X1, _, _ = O.spectral_input(900, 41, "normal")
+    X2, _, _ = O.spectral_input(700, 42, "uniform")
+    a, b = to_dev(X1, torch.float32), to_dev(X2, torch.float32)
+    handle.set_seed(9)
+    ref1 = P.project(handle, a, method="mixed")
+    handle.set_seed(9)
+    ref2 = P.project(handle, b, method="mixed")
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        o1, o2 = torch.empty_like(a), torch.empty_like(b)
+        handle.set_seed(9)
+        with torch.cuda.stream(s1):
+            P.project(handle, a, out=o1, method="mixed", synchronize=0)
+        with torch.cuda.stream(s2):
+            P.project(handle, b, out=o2, method="mixed", synchronize=0)
+        torch.cuda.synchronize()
+        assert torch.equal(o1, ref1) and torch.equal(o2, ref2)

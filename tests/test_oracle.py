@@ -214,6 +214,52 @@ def test_lanczos_matches_rayleigh_ritz(lam, iters):
     assert abs(up_textbook - up) > 1e-3 * up
 
 
+@pytest.mark.parametrize("lam", [-10.0, 3.0, 0.5])
+@pytest.mark.parametrize("iters", [2, 3, 4])
+def test_lanczos_tight_matches_rayleigh_ritz(lam, iters):
+    # The tight mode's residual is that of the Rayleigh-Ritz pair of
+    # B = A^T A itself; the same independent Krylov/QR route gives it.
+    rng = np.random.default_rng(7)
+    d = np.concatenate([[lam], rng.uniform(-abs(lam) / 2, abs(lam) / 2, 39)])
+    Qr, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    A = (Qr * d) @ Qr.T
+    q0 = np.ones(40)
+    B = A.T @ A
+    K = np.stack([np.linalg.matrix_power(B, j) @ q0 for j in range(iters)], axis=1)
+    Qk, _ = np.linalg.qr(K)
+    w, U = np.linalg.eigh(Qk.T @ B @ Qk)
+    y = Qk @ U[:, -1]
+    up_tb = np.sqrt(w[-1] + np.linalg.norm(B @ y - w[-1] * y))
+    lo, up = O.lanczos_two_norm(A, iters, 0.0, q0=q0, mode="tight")
+    assert lo == pytest.approx(np.sqrt(w[-1]), rel=1e-9)
+    assert up == pytest.approx(up_tb, rel=1e-7)
+
+
+def test_lanczos_tight_complete_krylov_space():
+    # A Krylov space that holds every eigenvector of B (n = 5, 50 iterations):
+    # theta = ||A||_2^2 and the Ritz residual vanishes, so the tight bound is
+    # ||A||_2 itself; the reference residual A y - theta y of the same pair is
+    # |lambda - lambda^2| and leaves up far above.
+    k = KATS["lanczos_5x5"]
+    A = np.array(k["input_colmajor"]).reshape(5, 5, order="F")
+    lo, up = O.lanczos_two_norm(A, 50, 0.0, q0=np.ones(5), mode="tight")
+    assert up == pytest.approx(k["two_norm"], rel=1e-7)
+    assert lo == pytest.approx(k["two_norm"], rel=1e-9)
+    _, up_ref = O.lanczos_two_norm(A, 50, 0.0, q0=np.ones(5))
+    nrm = k["two_norm"]
+    assert up_ref == pytest.approx(np.sqrt(nrm ** 2 + abs(nrm - nrm ** 2)), rel=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "normal", "cfg3"])
+def test_lanczos_tight_brackets_norm(kind):
+    X, Q, lam = O.spectral_input(300, 11, kind)
+    nrm = np.abs(lam).max()
+    lo, up = O.lanczos_two_norm(X, 20, 1e-10, mode="tight")
+    _, up_ref = O.lanczos_two_norm(X, 20, 1e-10)
+    assert lo <= nrm * (1 + 1e-12) <= up
+    assert up <= up_ref
+
+
 def test_lanczos_zero_matrix():
     assert O.lanczos_two_norm(np.zeros((6, 6))) == (0.0, 1.0)
 
