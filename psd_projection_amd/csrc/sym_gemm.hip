@@ -450,10 +450,40 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
   _Pragma("unroll") for (int p = 0; p < P; ++p)                                 \
   _Pragma("unroll") for (int i = 0; i < LOADS; ++i)                             \
     *(u32x4*)(smem + (buf) * TL::STAGE + (g * P + p) * TILE + soff[i]) = rg[g][p][i];
+  // FP32 operands: every 16x16x4 MFMA rounds the running sum once, i.e. n/4
+  // roundings at the sum's full magnitude per entry (the 16-bit MFMAs add 16
+  // products per rounding).  Two-level accumulation: FLUSH slabs (128 k) go
+  // into a zeroed partial sum that is then added to the total, so an entry
+  // sees 32 roundings at the partial's magnitude plus n/128 at its own
+  // (measured rms error of the float32 product at n = 4096 was 2.8x that of
+  // the 16-bit formats on exactly representable data before this).
+  constexpr bool TWO = FMT == FMT_F32 && !AVG;
+  constexpr int FLUSH = 4;
+  f32x4 part[SUB][SUB];
+  auto clear_part = [&]() {
+#pragma unroll
+    for (int a = 0; a < SUB; ++a)
+#pragma unroll
+      for (int b = 0; b < SUB; ++b) part[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  auto flush_part = [&]() {
+#pragma unroll
+    for (int a = 0; a < SUB; ++a)
+#pragma unroll
+      for (int b = 0; b < SUB; ++b) {
+        acc[0][a][b] += part[a][b];
+        part[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+  if constexpr (TWO) clear_part();
 #define PSDP_MMA(buf)                                                                          \
   {                                                                                            \
     const E* base_ = smem + (buf) * TL::STAGE;                                                 \
-    mma_slab<FMT, SUB>(base_, base_ + P * TILE, arow, brow, lane, acc[0]);                     \
+    if constexpr (TWO) {                                                                       \
+      mma_slab<FMT, SUB>(base_, base_ + P * TILE, arow, brow, lane, part);                     \
+    } else {                                                                                   \
+      mma_slab<FMT, SUB>(base_, base_ + P * TILE, arow, brow, lane, acc[0]);                   \
+    }                                                                                          \
     if constexpr (AVG)                                                                         \
       mma_slab<FMT, SUB>(base_ + 2 * P * TILE, base_ + 3 * P * TILE, arow, brow, lane, acc[NP - 1]); \
   }
@@ -463,10 +493,14 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
   for (int kt = 0; kt < KT - 1; ++kt) {
     PSDP_GLOAD((kt + 1) * BK)
     PSDP_MMA(kt & 1)
+    if constexpr (TWO) {
+      if ((kt + 1) % FLUSH == 0) flush_part();
+    }
     PSDP_LSTORE((kt + 1) & 1)
     __syncthreads();
   }
   PSDP_MMA((KT - 1) & 1)
+  if constexpr (TWO) flush_part();
   __syncthreads();
 #undef PSDP_GLOAD
 #undef PSDP_LSTORE
@@ -673,14 +707,38 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   }
 
+  // Three-piece (x9) products add 9 MFMAs per k-step into one sum, each
+  // rounding it at full magnitude; they accumulate every 4 slabs (128 k) into
+  // a zeroed partial sum that is then added to the total (two-level
+  // accumulation: n/128 roundings at the total's magnitude instead of 9 n/16).
+  constexpr bool TWO = P == 3;
+  constexpr int FLUSH = 4;  // slabs per partial sum
+  static_assert(!TWO || BK == 32, "x9 partial sums cover 128 k (same order for every tile size)");
+  f32x16 part[M][M];
+  if constexpr (TWO) {
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+      for (int b = 0; b < M; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) part[a][b][e] = 0.f;
+  }
+  auto flush_part = [&]() {
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+      for (int b = 0; b < M; ++b) {
+        acc[a][b] += part[a][b];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) part[a][b][e] = 0.f;
+      }
+  };
+
   // MFMAs of column block mj of k-step ks
 #define PSDP_COLUMN_MMA(ks, mj)                                                                   \
   _Pragma("unroll") for (int mi = 0; mi < M; ++mi) {                                              \
-    f32x16 c = acc[mi][mj];                                                                       \
-    if constexpr (P == 2) { /* x3: lo*hi, hi*lo, then hi*hi */                                    \
-      c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                           \
-      c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                           \
-    } else if constexpr (P == 3) { /* x9: smallest terms first */                                 \
+    if constexpr (P == 3) { /* x9: smallest terms first, into the slab's partial sum */           \
+      f32x16 c = part[mi][mj];                                                                    \
       c = mfma32(fa[ks][2][mi], fb[ks][2][mj], c, FMT);                                           \
       c = mfma32(fa[ks][1][mi], fb[ks][2][mj], c, FMT);                                           \
       c = mfma32(fa[ks][2][mi], fb[ks][1][mj], c, FMT);                                           \
@@ -689,8 +747,15 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
       c = mfma32(fa[ks][2][mi], fb[ks][0][mj], c, FMT);                                           \
       c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                           \
       c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                           \
+      part[mi][mj] = mfma32(fa[ks][0][mi], fb[ks][0][mj], c, FMT);                                \
+    } else {                                                                                      \
+      f32x16 c = acc[mi][mj];                                                                     \
+      if constexpr (P == 2) { /* x3: lo*hi, hi*lo, then hi*hi */                                  \
+        c = mfma32(fa[ks][1][mi], fb[ks][0][mj], c, FMT);                                         \
+        c = mfma32(fa[ks][0][mi], fb[ks][1][mj], c, FMT);                                         \
+      }                                                                                           \
+      acc[mi][mj] = mfma32(fa[ks][0][mi], fb[ks][0][mj], c, FMT);                                 \
     }                                                                                             \
-    acc[mi][mj] = mfma32(fa[ks][0][mi], fb[ks][0][mj], c, FMT);                                   \
   }
 
   constexpr int NREAD = 2 * KS * P * M;  // LDS reads per slab
@@ -728,6 +793,9 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
       __builtin_amdgcn_sched_group_barrier(0x008, MR, 0);  // MFMA
     }
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (TWO) {
+      if ((kt + 1) % FLUSH == 0) flush_part();
+    }
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -740,6 +808,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int mj = 0; mj < M; ++mj) { PSDP_COLUMN_MMA(ks, mj) }
+    if constexpr (TWO) flush_part();
   }
 #undef PSDP_COLUMN_MMA
   __builtin_amdgcn_sched_barrier(0);  // keep the epilogue's accumulator reads after the MFMAs
@@ -978,17 +1047,50 @@ __global__ __launch_bounds__(256) void sym_prod_grouped_kernel(const ProdDesc* _
   }
 }
 
+// Float32 tiles keep two workgroups per CU (8 waves): their two-level sums
+// (total + partial accumulators) must then fit 256 registers per wave, which
+// the compiler is held to here (without the bound it took 274 and ran one
+// workgroup per CU, 19% slower at n = 16384).
+template <int SUB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void sym_prod_f32_kernel(
+    ProdDesc d, const TileRef* __restrict__ tiles) {
+  using TL = Tiling<FMT_F32, SUB, false>;
+  __shared__ __attribute__((aligned(16))) typename TL::E smem[TL::SMEM_ELEMS];
+  const TileRef t = tiles[blockIdx.x];
+  if (t.bi < 0) return;
+  sym_tile<FMT_F32, SUB, false>(d, t.bi, t.bj, smem);
+}
+
+template <int SUB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void sym_prod_f32_grouped_kernel(
+    const ProdDesc* __restrict__ descs, const TileRef* __restrict__ tiles, int ntiles) {
+  using TL = Tiling<FMT_F32, SUB, false>;
+  __shared__ __attribute__((aligned(16))) typename TL::E smem[TL::SMEM_ELEMS];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    if (tr.mat < 0) continue;
+    const ProdDesc d = descs[tr.mat];
+    sym_tile<FMT_F32, SUB, false>(d, tr.bi, tr.bj, smem);
+  }
+}
+
 template <int FMT, int SUB, bool AVG>
 static hipError_t launch_one(const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
-  hipLaunchKernelGGL((sym_prod_kernel<FMT, SUB, AVG>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+  if constexpr (FMT == FMT_F32 && !AVG)
+    hipLaunchKernelGGL((sym_prod_f32_kernel<SUB>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+  else
+    hipLaunchKernelGGL((sym_prod_kernel<FMT, SUB, AVG>), dim3(ntiles), dim3(256), 0, s, d, tiles);
   return hipGetLastError();
 }
 
 template <int FMT, int SUB, bool AVG>
 static hipError_t launch_grouped(const ProdDesc* descs, const TileRef* tiles, int ntiles, int grid,
                                  hipStream_t s) {
-  hipLaunchKernelGGL((sym_prod_grouped_kernel<FMT, SUB, AVG>), dim3(grid), dim3(256), 0, s, descs, tiles,
-                     ntiles);
+  if constexpr (FMT == FMT_F32 && !AVG)
+    hipLaunchKernelGGL((sym_prod_f32_grouped_kernel<SUB>), dim3(grid), dim3(256), 0, s, descs, tiles, ntiles);
+  else
+    hipLaunchKernelGGL((sym_prod_grouped_kernel<FMT, SUB, AVG>), dim3(grid), dim3(256), 0, s, descs, tiles,
+                       ntiles);
   return hipGetLastError();
 }
 
