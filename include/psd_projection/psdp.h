@@ -168,6 +168,18 @@ psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int6
                                const psdp_config_t* cfg, void* ws, size_t ws_bytes);
 psdp_status_t psdp_project_f32(psdp_handle_t h, const float* X, float* Y, int64_t n,
                                const psdp_config_t* cfg, void* ws, size_t ws_bytes);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory inside a call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): psdp_project_f32 with its workspace,
+ * Lanczos state and tile lists taken from a fixed 64 MiB device array that the
+ * library holds on every GPU it runs on until the process ends; the call
+ * makes no hipMalloc/hipFree.  Sizes whose workspace does not fit are
+ * rejected with PSDP_ERR_INVALID_ARG (n up to 1664 for the mixed schedule,
+ * 1920 for the FP32 and FP16 methods, 1408 for the emulated one).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (any
+ * handle, any stream) race. */
+psdp_status_t psdp_project_f32_noalloc(psdp_handle_t h, const float* X, float* Y, int64_t n,
+                                       const psdp_config_t* cfg);
 
 /* One symmetric product of X, Y (symmetric n x n float32, column-major) with
  * operands rounded to `format`: C = upper triangle of X Y mirrored, or with

@@ -126,6 +126,8 @@ _PROTOS = {
                                         ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_size_t]),
     "psdp_project_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_size_t]),
+    "psdp_project_f32_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                ctypes.POINTER(Config)]),
     "psdp_sym_product_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "psdp_project_batched_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Block), ctypes.c_int,
@@ -587,6 +589,22 @@ def project(handle, X, out=None, method="mixed", config: Config | None = None, w
     return out
 
 
+def project_noalloc(handle, X, out=None, method="mixed", config: Config | None = None, **overrides):
+    """psdp_project_f32_noalloc: Y = Pi(X) for a float32 matrix with no device
+    allocation in the call (workspace from the library's per-GPU scratch
+    array; calls that overlap on one GPU race)."""
+    import torch
+    handle._sync_stream()
+    n = _size(X, None)
+    if X.dtype != torch.float32:
+        raise TypeError("expected a float32 matrix")
+    if out is None:
+        out = torch.empty_like(X)
+    cfg = config if config is not None else default_config(method, **overrides)
+    _check(lib().psdp_project_f32_noalloc(handle.ptr, _ptr(X), _ptr(out), n, ctypes.byref(cfg)))
+    return out
+
+
 def workspace_size(n, method="mixed", **overrides) -> int:
     cfg = default_config(method, **overrides)
     b = ctypes.c_size_t()
@@ -665,5 +683,5 @@ __all__ = [
     "composite_FP32_emulated", "composite_FP32_emulated_auto_scale", "approximate_two_norm",
     "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "project_packed", "sym_product", "workspace_size",
     "compute_eigenpairs", "lobpcg", "composite_FP32_auto_scale_deflate", "composite_FP16_auto_scale_deflate",
-    "eig_FP64_deflate", "project_method", "eig_psd_alloc", "free_device",
+    "eig_FP64_deflate", "project_method", "eig_psd_alloc", "free_device", "project_noalloc",
 ]

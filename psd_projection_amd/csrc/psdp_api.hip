@@ -50,6 +50,8 @@ hipError_t launch_sym_pad_batched(const PadJob* jobs, const TileRef* tiles, int 
 hipError_t launch_scale_split_batched(const SplitJob* jobs, const int2* chunks, int nchunks, int fmt_a, int fmt_x,
                                       hipStream_t st);
 size_t eig_workspace_bytes(int n, int f64);
+psdp_status_t noalloc_scratch(int device, void** ptr, size_t* bytes);
+hipError_t launch_fill_tile_list(TileRef* out, int T, hipStream_t s);
 int eig_project(void* rb_handle, hipStream_t s, double* X, int n, int f64, void* ws, int* info_dev, double* eig);
 void* eig_create_blas();
 void eig_destroy_blas(void* rb);
@@ -232,6 +234,8 @@ struct psdp_handle_st {
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
   std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, TAIL_SUB tail)
   int cus = 256;                                  // compute units of the device
+  void* scratch = nullptr;                        // this device's scratch array (psdp_project_f32_noalloc)
+  size_t scratch_bytes = 0;
   // pinned host staging ring for batched descriptors: slot k is reused only
   // after the copy recorded in ev[k] has run
   static constexpr int NSTAGE = 4;
@@ -582,11 +586,23 @@ static ProdDesc base_desc(const Plan& p) {
   return d;
 }
 
+// Tile lists that live in caller-provided device memory (the allocation-free
+// entry point builds them there), by tile sub-size.
+struct FixedTiles {
+  const TileRef* list[9] = {};
+  int count[9] = {};
+};
+
 // Enqueues the whole projection.
 // One symmetric product on tiles of 32 * sub: sub 8 runs whole rounds of
 // 256 x 256 tiles and re-cuts a partial last round into 128 x 128 tiles.
 static psdp_status_t launch_product(psdp_handle_t h, const Plan& p, int fmt, int avg, const ProdDesc& d, int sub,
-                                    hipStream_t s) {
+                                    hipStream_t s, const FixedTiles* ft = nullptr) {
+  if (ft) {
+    if (sub < 0 || sub > 8 || !ft->list[sub]) return fail(PSDP_ERR_INVALID_ARG, "internal: no tile list for %d", sub);
+    HIPCHK(launch_sym_product(fmt, sub, avg, d, ft->list[sub], ft->count[sub], s));
+    return PSDP_OK;
+  }
   if (sub == 8) {
     TileList big, small;
     STCHK(get_split_tiles(h, p.npad / 256, big, small));
@@ -611,15 +627,19 @@ static int tuned_sub(psdp_handle_t h, const Plan& p, int fmt, int avg) {
 
 // fixed_scale: s of the entry points without the bound (1 for the reference's
 // composite_* functions, the deflated matrix's scale for the *_deflate ones).
+// st: device Lanczos state (default: the handle's); ft: tile lists in caller
+// memory (default: the handle's cached lists).
 static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
-                                        void* Y, int y_f64, const psdp_config_t* cfg, double fixed_scale = 1.0) {
+                                        void* Y, int y_f64, const psdp_config_t* cfg, double fixed_scale = 1.0,
+                                        LanczosState* st = nullptr, const FixedTiles* ft = nullptr) {
   hipStream_t s = h->stream;
   const Schedule& sch = p.sch;
+  if (!st) st = h->st;
   // one product: tile size by format, then the launch
   auto product = [&](int fmt, int avg, const ProdDesc& d) -> psdp_status_t {
     DESCCHK(d, fmt);
     ProfScope ps(h, PSDP_PROF_PROD_F32 + fmt);
-    return launch_product(h, p, fmt, avg, d, tuned_sub(h, p, fmt, avg), s);
+    return launch_product(h, p, fmt, avg, d, tuned_sub(h, p, fmt, avg), s, ft);
   };
 
   {
@@ -632,13 +652,13 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     const int iters = std::min(p.lanczos_iters, p.n);  // max_iter = min(max_iter, n), src/lanczos.cu:55
     ProfScope ps(h, PSDP_PROF_LANCZOS);
     HIPCHK(launch_lanczos(b.a32, 0, p.ld, p.n, p.npad, iters, tol, h->seed, margin,
-                          cfg->bound_mode == PSDP_BOUND_TIGHT, h->st, b.v, s));
+                          cfg->bound_mode == PSDP_BOUND_TIGHT, st, b.v, s));
   } else {
-    HIPCHK(launch_set_scale(h->st, fixed_scale, s));
+    HIPCHK(launch_set_scale(st, fixed_scale, s));
   }
   {
     ProfScope ps(h, PSDP_PROF_PREP);
-    HIPCHK(launch_scale_split(b.a32, p.N, &h->st->inv_scale, sch.fmt[0], b.ap, sch.final_fmt, b.x32, b.xp, s));
+    HIPCHK(launch_scale_split(b.a32, p.N, &st->inv_scale, sch.fmt[0], b.ap, sch.final_fmt, b.x32, b.xp, s));
   }
 
   for (int i = 0; i < sch.nsteps; ++i) {
@@ -685,8 +705,8 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   d4.out = Y;
   d4.out_ld = p.n;
   d4.out_f64 = y_f64;
-  d4.scale = &h->st->scale;
-  d4.nonfinite = &h->st->out_nonfinite;
+  d4.scale = &st->scale;
+  d4.nonfinite = &st->out_nonfinite;
   STCHK(product(sch.final_fmt, sch.final_avg, d4));
   return PSDP_OK;
 }
@@ -699,10 +719,10 @@ static void clear_profile(psdp_handle_t h) {
   h->prof.clear();
 }
 
-static psdp_status_t check_flags(psdp_handle_t h) {
+static psdp_status_t check_flags(psdp_handle_t h, const LanczosState* dev_state = nullptr) {
   LanczosState st;
   HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&st, dev_state ? dev_state : h->st, sizeof st, hipMemcpyDeviceToHost));
   if (st.nonfinite) return fail(PSDP_ERR_NONFINITE, "non-finite value in the input or the spectral bound");
   if (st.out_nonfinite)
     return fail(PSDP_ERR_NONFINITE,
@@ -799,6 +819,15 @@ psdp_status_t psdp_create(psdp_handle_t* out, int device) {
     return fail(PSDP_ERR_NOMEM, "state allocation failed");
   }
   (void)hipMemset(h->st, 0, sizeof(LanczosState));
+  // the device address of this device's scratch array, looked up once per
+  // device with the device current (the allocation-free entry point uses it)
+  const psdp_status_t sst = noalloc_scratch(device, &h->scratch, &h->scratch_bytes);
+  if (sst != PSDP_OK) {
+    (void)hipFree(h->st);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return sst;
+  }
   *out = h;
   return PSDP_OK;
 }
@@ -1053,6 +1082,71 @@ psdp_status_t psdp_project_f64(psdp_handle_t h, const double* X, double* Y, int6
 psdp_status_t psdp_project_f32(psdp_handle_t h, const float* X, float* Y, int64_t n, const psdp_config_t* cfg,
                                void* ws, size_t ws_bytes) {
   return project(h, X, 0, Y, 0, n, cfg, ws, ws_bytes);
+}
+
+// psdp_project_f32 without any device allocation: Lanczos state, tile lists
+// and workspace come from the device's scratch array (layout: state, then
+// the tile lists, then the workspace of the plan), whose address the handle
+// looked up at creation.  Same kernels, tile sizes and order of work as
+// psdp_project_f32 (tile lists in column order instead of the XCD-aware
+// order, which changes no result).  Nothing here needs to start at zero:
+// every buffer is written before it is read, as in the handle's arena.
+static constexpr size_t NOALLOC_HEAD = 64 * 1024;  // state + tile lists
+psdp_status_t psdp_project_f32_noalloc(psdp_handle_t h, const float* X, float* Y, int64_t n,
+                                       const psdp_config_t* cfg) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  if (!X || !Y) return fail(PSDP_ERR_INVALID_ARG, "null matrix pointer");
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->scratch) return fail(PSDP_ERR_DEVICE, "the handle has no scratch array");
+  Plan p;
+  STCHK(make_plan(cfg, n, p));
+  const size_t room = h->scratch_bytes - NOALLOC_HEAD;
+  if (p.total > room)
+    return fail(PSDP_ERR_INVALID_ARG,
+                "n = %lld needs %zu bytes of workspace; the allocation-free entry point has %zu (use "
+                "psdp_project_f32 for this size)",
+                (long long)n, p.total, room);
+  char* base = (char*)h->scratch;
+  LanczosState* st = (LanczosState*)base;
+  static_assert(sizeof(LanczosState) <= 4096, "state fits its slot");
+  // one tile list per tile size the plan's products use
+  FixedTiles ft;
+  TileRef* lists = (TileRef*)(base + 4096);
+  size_t used = 0;
+  auto need = [&](int fmt, int avg) -> psdp_status_t {
+    const int sub = tuned_sub(h, p, fmt, avg);
+    if (sub != 2 && sub != 4)
+      return fail(PSDP_ERR_UNSUPPORTED, "tile size %d is not available to the allocation-free entry point", 32 * sub);
+    if (ft.list[sub]) return PSDP_OK;
+    const int T = p.npad / (32 * sub);
+    const size_t cnt = (size_t)T * (T + 1) / 2;
+    if (4096 + (used + cnt) * sizeof(TileRef) > NOALLOC_HEAD)
+      return fail(PSDP_ERR_INVALID_ARG, "internal: tile lists exceed their area");
+    HIPCHK(launch_fill_tile_list(lists + used, T, h->stream));
+    ft.list[sub] = lists + used;
+    ft.count[sub] = (int)cnt;
+    used += cnt;
+    return PSDP_OK;
+  };
+  for (int i = 0; i < p.sch.nsteps; ++i) {
+    STCHK(need(p.sch.fmt[i], 0));
+    STCHK(need(p.sch.fmt[i], p.sch.avg[i]));
+  }
+  STCHK(need(p.sch.final_fmt, p.sch.final_avg));
+  Bufs b = carve(p, base + NOALLOC_HEAD);
+  const bool prof = h->profiling;
+  h->profiling = false;  // timing events are created per launch
+  const psdp_status_t est = enqueue_projection(h, p, b, X, 0, Y, 0, cfg, 1.0, st, &ft);
+  h->profiling = prof;
+  STCHK(est);
+  if (cfg->synchronize) STCHK(check_flags(h, st));
+  return PSDP_OK;
 }
 
 static psdp_status_t dropin(psdp_handle_t h, double* mat, int64_t n, void* ws, size_t ws_bytes, int method,
