@@ -404,6 +404,92 @@ def eig_FP64_psd(X, return_eigenvalues=False):
     return Y
 
 
+def _scipy_lapack():
+    """ctypes handle on the LAPACK that scipy.linalg uses (its bundled
+    OpenBLAS, whose symbols carry the prefix scipy_)."""
+    import ctypes
+    import scipy.linalg  # noqa: F401
+    from threadpoolctl import threadpool_info
+    paths = [i["filepath"] for i in threadpool_info()
+             if i.get("internal_api") == "openblas" and "scipy.libs" in i.get("filepath", "")]
+    if not paths:
+        raise RuntimeError("scipy's OpenBLAS not found")
+    return ctypes.CDLL(paths[0]), paths[0]
+
+
+def eig_FP64_psd_staged(A_in, log=None):
+    """eig_FP64_psd (src/eig_FP64_psd.cu:24-85) on the host: LAPACK dsyevd
+    (jobz = V, uplo = U) run as its three stages -- dsytrd (reduction to
+    tridiagonal form), dstedc (divide and conquer with eigenvectors), dormtr
+    (back-transformation) -- with the optimal workspaces LAPACK reports, from
+    the LAPACK that scipy.linalg uses; then the clamp and V diag(max(w, 0)) V^T
+    (src/eig_FP64_psd.cu:51-75).  This is the timed CPU baseline; cuSOLVER's
+    Dsyevd also runs with its queried workspace (src/eig_FP64_psd.cu:34-43).
+    log(msg) receives each stage's time when it ends.  Returns (Y, w ascending,
+    stage seconds)."""
+    import ctypes
+    import numpy as np
+    import time
+    L, _ = _scipy_lapack()
+    n = A_in.shape[0]
+    A = np.array(A_in, dtype=np.float64, order="F", copy=True)
+    c_int, c_dbl = ctypes.c_int, ctypes.c_double
+    P_ = ctypes.POINTER
+    by = ctypes.byref
+    N = c_int(n)
+    info = c_int(0)
+    U, I, Lc, Nc = (ctypes.c_char(c) for c in (b"U", b"I", b"L", b"N"))
+    one = ctypes.c_size_t(1)
+    dptr = lambda a: a.ctypes.data_as(P_(c_dbl))
+    d = np.empty(n)
+    e = np.empty(max(n - 1, 1))
+    tau = np.empty(max(n - 1, 1))
+    q = np.empty(1)
+    stages = {}
+    say = log if log is not None else (lambda msg: None)
+
+    def check(name):
+        if info.value != 0:
+            raise RuntimeError(f"LAPACK {name} info {info.value}")
+
+    L.scipy_dsytrd_(by(U), by(N), dptr(A), by(N), dptr(d), dptr(e), dptr(tau), dptr(q), by(c_int(-1)), by(info), one)
+    work = np.empty(max(int(q[0]), 1))
+    t = time.perf_counter()
+    L.scipy_dsytrd_(by(U), by(N), dptr(A), by(N), dptr(d), dptr(e), dptr(tau), dptr(work), by(c_int(work.size)),
+                    by(info), one)
+    stages["dsytrd"] = time.perf_counter() - t
+    check("dsytrd")
+    say(f"eig_FP64_psd_staged n={n}: dsytrd {stages['dsytrd']:.1f} s")
+    Z = np.empty((n, n), order="F")
+    qi = c_int(0)
+    L.scipy_dstedc_(by(I), by(N), dptr(d), dptr(e), dptr(Z), by(N), dptr(q), by(c_int(-1)), by(qi), by(c_int(-1)),
+                    by(info), one)
+    work = np.empty(max(int(q[0]), 1))
+    iwork = np.empty(max(qi.value, 1), dtype=np.int32)
+    t = time.perf_counter()
+    L.scipy_dstedc_(by(I), by(N), dptr(d), dptr(e), dptr(Z), by(N), dptr(work), by(c_int(work.size)),
+                    iwork.ctypes.data_as(P_(c_int)), by(c_int(iwork.size)), by(info), one)
+    stages["dstedc"] = time.perf_counter() - t
+    check("dstedc")
+    say(f"eig_FP64_psd_staged n={n}: dstedc {stages['dstedc']:.1f} s")
+    L.scipy_dormtr_(by(Lc), by(U), by(Nc), by(N), by(N), dptr(A), by(N), dptr(tau), dptr(Z), by(N), dptr(q),
+                    by(c_int(-1)), by(info), one, one, one)
+    work = np.empty(max(int(q[0]), 1))
+    t = time.perf_counter()
+    L.scipy_dormtr_(by(Lc), by(U), by(Nc), by(N), by(N), dptr(A), by(N), dptr(tau), dptr(Z), by(N), dptr(work),
+                    by(c_int(work.size)), by(info), one, one, one)
+    stages["dormtr"] = time.perf_counter() - t
+    check("dormtr")
+    say(f"eig_FP64_psd_staged n={n}: dormtr {stages['dormtr']:.1f} s")
+    del A, work
+    t = time.perf_counter()
+    Y = (Z * np.maximum(d, 0.0)) @ Z.T
+    stages["clamp_gemm"] = time.perf_counter() - t
+    say(f"eig_FP64_psd_staged n={n}: clamp + V diag V^T {stages['clamp_gemm']:.1f} s")
+    order = np.argsort(d)  # dstedc returns ascending eigenvalues already
+    return Y, d[order], stages
+
+
 def eig_FP32_psd(X, return_eigenvalues=False):
     """src/eig_FP32_psd.cu:23-96: FP64 -> FP32, ssyevd, clamp, FP32 V diag V^T."""
     import scipy.linalg as sla
@@ -733,6 +819,8 @@ PINNING = {
                         "tight mode: a complete Krylov space gives up = ||A||_2 to 1e-12 (the reference "
                         "residual of the same pair does not), up >= ||A||_2 >= lo on random spectra",
     "eig_FP64_psd": "diag KAT, PSD fixed point, closed form Q max(L,0) Q^T",
+    "eig_FP64_psd_staged": "equals eig_FP64_psd (scipy eigh, driver evd) and the closed form on random and "
+                           "diagonal inputs; eigenvalues equal numpy eigvalsh",
     "GlibcRandom": "first outputs after srand(1) (1804289383, 846930886, ...) and libc rand()",
     "round_e4m3": "OCP E4M3FN grid KATs (448, 2^-6, 2^-9, ties), brute force over all 256 codes, "
                   "torch float8_e4m3fn casts in range",

@@ -386,3 +386,20 @@ def test_e4m3_product_error_band():
     e = np.linalg.norm(O.emulated_product(X, X, "e4m3") - exact) / np.linalg.norm(exact)
     # 3 mantissa bits: relative operand error <= 2^-4, product error a few 1e-2
     assert 5e-3 < e < 6e-2, e
+
+
+@pytest.mark.parametrize("kind", ["normal", "uniform"])
+def test_eig_fp64_staged_matches_eigh(kind):
+    # The timed CPU baseline (dsytrd + dstedc + dormtr, the stages of dsyevd)
+    # equals the single-call dsyevd route and the closed form.
+    X, Q, lam = O.spectral_input(257, 8, kind)
+    Y, w, stages = O.eig_FP64_psd_staged(X)
+    Y2, w2 = O.eig_FP64_psd(X, return_eigenvalues=True)
+    assert set(stages) == {"dsytrd", "dstedc", "dormtr", "clamp_gemm"}
+    assert np.abs(Y - Y2).max() < 1e-12
+    assert np.abs(w - w2).max() < 1e-12
+    assert O.rel_fro_error(Y, O.closed_form_projection(Q, lam)) < 1e-12
+    k = KATS["simple5"]
+    Y5, w5, _ = O.eig_FP64_psd_staged(np.diag(k["diagonal"]))
+    assert np.abs(Y5 - np.diag(k["expected_diagonal"])).max() < 1e-14
+    assert np.allclose(w5, np.sort(k["diagonal"]))
