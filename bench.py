@@ -354,12 +354,13 @@ def bench_served(args, P, W, torch):
         ns = rng.choice(sizes, size=count, p=weights)
         return [Request(i, int(n), pool[int(n)][i % 4][0], 0.0) for i, n in enumerate(ns)]
 
-    ex = TorchExecutor(0, args.method, small_lanes=1, large_lanes=2, seed=5)
+    ex = TorchExecutor(0, args.method, large_lanes=args.serve_lanes, seed=5, session_bytes=8 << 30,
+                       ticks_ahead=args.serve_ticks_ahead)
     ex.warm(sizes)
     budget = 64 << 30
     # capacity: a burst (everything arrives at once)
     burst = make(min(1000, args.serve_requests))
-    srv = ProjectionServer(ex, budget)
+    srv = ProjectionServer(ex, budget, on_done=lambda r: None)  # results dropped (released) as they complete
     t0 = time.perf_counter()
     srv.run(burst, [0.0] * len(burst))
     torch.cuda.synchronize()
@@ -372,9 +373,8 @@ def bench_served(args, P, W, torch):
 
     def on_done(r):
         X, Pm, pn = pool[r.n][r.rid % 4]
-        with torch.cuda.stream(ex.stream_of(r.lane)):
+        with torch.cuda.stream(ex.stream_of(r)):
             errs.append((r.Y.double() - Pm).norm() / pn)
-        r.Y = None
 
     srv = ProjectionServer(ex, budget, on_done=on_done)
     done = srv.run(reqs, arrivals)
@@ -383,10 +383,11 @@ def bench_served(args, P, W, torch):
     e = torch.stack(errs).cpu().numpy() if errs else np.zeros(1)
     res = {"workload": f"{len(reqs)} requests, Poisson at {rate:.1f}/s = {args.serve_load} x "
                        f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}; "
-                       f"lanes: 1 small (grouped batches of n <= 1024) + 2 large",
+                       f"n <= 1024 in one continuous-batching session (requests join between products), "
+                       f"n > 1024 on {args.serve_lanes} lane(s)",
            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
            "proj_per_s": round(st["proj_per_s"], 1), "capacity_proj_per_s": round(cap, 1),
-           "launches": srv.batches, "peak_admitted_bytes": srv.peak_used,
+           "lane_launches": srv.batches, "peak_held_bytes": srv.peak_used, "admission_refusals": srv.refusals,
            "max_rel_err": float(e.max()), "requests_over_1e-3": int((e > 1e-3).sum())}
     log(f"served: {res}")
     return res
@@ -410,6 +411,9 @@ def main():
     ap.add_argument("--small-configs", type=int, default=1,
                     help="configs 1 (n=512) and 2 (n=8192, FP32 vs FP16 chain) (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
+    ap.add_argument("--serve-lanes", type=int, default=1, help="lanes (streams) for the n > 1024 requests")
+    ap.add_argument("--serve-ticks-ahead", type=int, default=4,
+                    help="session ticks the served loop keeps enqueued ahead of the GPU")
     ap.add_argument("--deflate", type=int, default=1024, help="size of the deflated-projection leg (0: skip)")
     ap.add_argument("--autotune", type=int, default=1, help="tune tile sizes before timing (0: built-in tiles)")
     ap.add_argument("--basis", default="haar", choices=("haar", "householder"),

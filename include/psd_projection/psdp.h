@@ -221,6 +221,38 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
  * caller, an SDP solver, include/psd_projection/check.h:10, loops over cones). */
 psdp_status_t psdp_project_packed_f32(psdp_handle_t h, const float* X, float* Y, const int64_t* sizes, int count,
                                       const psdp_config_t* cfg);
+/* ---- continuous batching: a session of blocks that join and leave between products ----
+ * No reference counterpart: the reference projects one matrix per call; its
+ * intended caller, an ADMM/SDP solver (include/psd_projection/check.h:10),
+ * projects many cone blocks every iteration.
+ * A session keeps a changing set of float32 blocks in flight on the handle's
+ * stream.  psdp_session_add admits a block (X, Y: device, column-major n x n;
+ * Y is written when the block completes) and returns its ticket; its
+ * workspace comes from the session's arena, sized once at creation, so
+ * nothing is allocated per block: PSDP_ERR_NOMEM says the arena has no room
+ * for it now (retry after completions), PSDP_ERR_INVALID_ARG that it never
+ * will.  Each psdp_session_step enqueues one tick: the prologue (symmetrize,
+ * Lanczos bound, scale/split) of the blocks added since the last tick, then
+ * ONE filter-chain product for every block past its prologue, one grouped
+ * launch per operand format and tile class.  A block thus starts at the next
+ * product boundary instead of waiting for the blocks ahead of it, and leaves
+ * after its output product; *in_flight_ticks (may be NULL) counts the ticks
+ * enqueued and not yet complete.  psdp_session_poll returns (up to max)
+ * tickets whose outputs are complete, *active (may be NULL): the blocks not
+ * yet complete, and *in_flight_ticks (may be NULL), without enqueueing.  A block's result is bitwise the result
+ * psdp_project_batched_f32 gives it alone.  cfg must auto-scale.  One host
+ * thread drives a session (through its handle). */
+typedef struct psdp_session_st* psdp_session_t;
+psdp_status_t psdp_session_create(psdp_handle_t h, const psdp_config_t* cfg, size_t arena_bytes,
+                                  psdp_session_t* session);
+psdp_status_t psdp_session_destroy(psdp_session_t session);
+/* Arena bytes a block of size n takes in this session. */
+psdp_status_t psdp_session_block_bytes(psdp_session_t session, int64_t n, size_t* bytes);
+psdp_status_t psdp_session_add(psdp_session_t session, const float* X, float* Y, int64_t n, int64_t* ticket);
+psdp_status_t psdp_session_step(psdp_session_t session, int* in_flight_ticks);
+psdp_status_t psdp_session_poll(psdp_session_t session, int64_t* tickets, int max, int* count, int* active,
+                                int* in_flight_ticks);
+
 /* Tile classes of the batched call: blocks with n >= big_block_n use
  * 128 x 128 tiles, the rest 64 x 64 (0: the built-in default); concurrent != 0
  * runs the two classes of each product on two streams. */

@@ -128,6 +128,16 @@ _PROTOS = {
                                         ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_size_t]),
     "psdp_project_f32_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                 ctypes.POINTER(Config)]),
+    "psdp_session_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Config), ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "psdp_session_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "psdp_session_block_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]),
+    "psdp_session_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "psdp_session_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "psdp_session_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "psdp_sym_product_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "psdp_project_batched_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Block), ctypes.c_int,
@@ -366,6 +376,62 @@ class Handle:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Session:
+    """psdp_session_t: continuous batching of float32 blocks on a handle's
+    stream.  add() admits a block (None when the arena has no room now);
+    step() enqueues the prologue of new blocks and one product of every ready
+    block, returns the ticks in flight; poll() returns (completed tickets,
+    blocks not yet complete).  A block's result equals project_batched alone."""
+
+    def __init__(self, handle, arena_bytes, method="mixed", config: Config | None = None, **overrides):
+        self.handle = handle
+        cfg = config if config is not None else default_config(method, **overrides)
+        self._s = ctypes.c_void_p()
+        _check(lib().psdp_session_create(handle.ptr, ctypes.byref(cfg), int(arena_bytes), ctypes.byref(self._s)))
+        self._ids = (ctypes.c_int64 * 4096)()
+
+    def block_bytes(self, n):
+        b = ctypes.c_size_t()
+        _check(lib().psdp_session_block_bytes(self._s, int(n), ctypes.byref(b)))
+        return b.value
+
+    def add(self, X, Y):
+        n = _size(X, None)
+        t = ctypes.c_int64()
+        st = lib().psdp_session_add(self._s, _ptr(X), _ptr(Y), n, ctypes.byref(t))
+        if st == ERR_NOMEM:
+            return None
+        _check(st)
+        return t.value
+
+    def step(self):
+        k = ctypes.c_int()
+        _check(lib().psdp_session_step(self._s, ctypes.byref(k)))
+        return k.value
+
+    def poll(self):
+        """(completed tickets, blocks not yet complete, ticks in flight)."""
+        c, a, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        out = []
+        while True:
+            _check(lib().psdp_session_poll(self._s, self._ids, len(self._ids), ctypes.byref(c), ctypes.byref(a),
+                                           ctypes.byref(k)))
+            out.extend(self._ids[i] for i in range(c.value))
+            if c.value < len(self._ids):
+                return out, a.value, k.value
+
+    def close(self):
+        if self._s:
+            lib().psdp_session_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _ws(workspace):
@@ -683,5 +749,5 @@ __all__ = [
     "composite_FP32_emulated", "composite_FP32_emulated_auto_scale", "approximate_two_norm",
     "eig_FP64_psd", "eig_FP32_psd", "project", "project_batched", "project_packed", "sym_product", "workspace_size",
     "compute_eigenpairs", "lobpcg", "composite_FP32_auto_scale_deflate", "composite_FP16_auto_scale_deflate",
-    "eig_FP64_deflate", "project_method", "eig_psd_alloc", "free_device", "project_noalloc",
+    "eig_FP64_deflate", "project_method", "eig_psd_alloc", "free_device", "project_noalloc", "Session",
 ]

@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -265,9 +266,9 @@ static psdp_status_t staging_slot(psdp_handle_t h, size_t bytes, char** out) {
 }
 
 // marks the current slot's copy (already enqueued on the handle's stream)
-static psdp_status_t staging_release(psdp_handle_t h) {
+static psdp_status_t staging_release(psdp_handle_t h, hipStream_t on = nullptr) {
   const int k = h->stage_next;
-  HIPCHK(hipEventRecord(h->stage_ev[k], h->stream));
+  HIPCHK(hipEventRecord(h->stage_ev[k], on ? on : h->stream));
   h->stage_next = (k + 1) % psdp_handle_st::NSTAGE;
   return PSDP_OK;
 }
@@ -1333,6 +1334,144 @@ static std::vector<TileRef> batched_tile_order(const std::vector<int>& blocks, i
   return out;
 }
 
+// Buffers of one block of a grouped projection, as byte offsets from the
+// workspace base (kNone: a buffer the schedule does not use).  Blocks are
+// small: rows are not padded beyond the tile (ld = npad).
+struct BlockBufs {
+  int n = 0, npad = 0, iters = 0;
+  size_t a32 = 0, p3 = 0, p2 = kNone, x32 = kNone, vec = 0;
+  size_t ap[3] = {kNone, kNone, kNone}, p2p[3] = {kNone, kNone, kNone};
+  size_t p3p[3] = {kNone, kNone, kNone}, xp[3] = {kNone, kNone, kNone};
+};
+
+// Tile edge of a block's grouped products: 128 from big_n, else 64 (a
+// function of the block's own size only, so its result never depends on
+// the rest of the batch).
+static int block_tile(int n, int big_n) { return n >= big_n ? 128 : 64; }
+
+// Lays out one block's buffers from byte offset `off`; returns the end.
+static size_t layout_block(const Schedule& sch, int n, int iters, int big_n, size_t off, BlockBufs& q) {
+  bool any_f32 = false;
+  int pa = pieces_of(sch.final_fmt), pp = 0;
+  for (int i = 0; i < sch.nsteps; ++i) {
+    any_f32 |= sch.fmt[i] == FMT_F32;
+    pa = std::max(pa, pieces_of(sch.fmt[i]));
+    pp = std::max(pp, pieces_of(sch.fmt[i]));
+  }
+  const int px = pieces_of(sch.final_fmt);
+  q = BlockBufs();
+  q.n = n;
+  const int bt = block_tile(n, big_n);
+  q.npad = (n + bt - 1) / bt * bt;
+  q.iters = std::min(iters, n);
+  const int64_t N = (int64_t)q.npad * q.npad;
+  const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
+  q.a32 = off, off += f32b;
+  q.p3 = off, off += f32b;
+  if (any_f32) q.p2 = off, off += f32b;
+  if (sch.final_fmt == FMT_F32) q.x32 = off, off += f32b;
+  for (int i = 0; i < pa; ++i) q.ap[i] = off, off += u16b;
+  for (int i = 0; i < pp; ++i) q.p2p[i] = off, off += u16b;
+  for (int i = 0; i < pp; ++i) q.p3p[i] = off, off += u16b;
+  for (int i = 0; i < px; ++i) q.xp[i] = off, off += u16b;
+  q.vec = off;
+  off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
+  return off;
+}
+
+// Launch descriptors of one block laid out by layout_block at `base`: its
+// product descriptor of stage s at pd[s * stride] (3 per filter step, then
+// the output product), its Lanczos, pad and scale/split jobs.  st: the
+// block's device Lanczos state.  Pure host code.
+static void block_descs(const Schedule& sch, const BlockBufs& q, char* base, const float* X, float* Y,
+                        LanczosState* st, ProdDesc* pd, size_t stride, LanczosJob& J, PadJob& pj, SplitJob& S) {
+  auto F = [&](size_t o) { return o != kNone ? (float*)(base + o) : (float*)nullptr; };
+  auto U = [&](size_t o) { return o != kNone ? (uint16_t*)(base + o) : (uint16_t*)nullptr; };
+  Bufs B;
+  B.a32 = F(q.a32);
+  B.p3 = F(q.p3);
+  B.p2 = F(q.p2);
+  B.x32 = F(q.x32);
+  for (int i = 0; i < 3; ++i) {
+    B.ap[i] = U(q.ap[i]);
+    B.p2p[i] = U(q.p2p[i]);
+    B.p3p[i] = U(q.p3p[i]);
+    B.xp[i] = U(q.xp[i]);
+  }
+  const size_t vb = align256((size_t)q.npad * 8);
+  char* v = base + q.vec;
+  J.A = B.a32;
+  J.ld = q.npad;
+  J.n = q.n;
+  J.m = q.npad;
+  J.max_iter = q.iters;
+  J.st = st;
+  J.v.q = (double*)(v + 0 * vb);
+  J.v.w = (double*)(v + 1 * vb);
+  J.v.w1 = (double*)(v + 2 * vb);
+  J.v.vold = (double*)(v + 3 * vb);
+  J.v.y = (double*)(v + 4 * vb);
+  J.v.ay = (double*)(v + 5 * vb);
+  J.v.V = (double*)(v + 6 * vb);
+  J.v.part = (double*)B.p3;  // GEMV partial sums; A^3 is written only after the bound
+  pj = PadJob{X, q.n, q.npad, B.a32};
+  S.a32 = B.a32;
+  S.count = (int64_t)q.npad * q.npad;
+  S.inv = &st->inv_scale;
+  S.x32 = B.x32;
+  for (int i = 0; i < 3; ++i) {
+    S.ap[i] = B.ap[i];
+    S.xp[i] = B.xp[i];
+  }
+  Plan p;
+  p.n = q.n;
+  p.npad = q.npad;
+  p.ld = q.npad;
+  for (int i = 0; i < sch.nsteps; ++i) {
+    const int f = sch.fmt[i];
+    const int fn = i + 1 < sch.nsteps ? sch.fmt[i + 1] : sch.final_fmt;
+    const bool last = i + 1 == sch.nsteps;
+    ProdDesc& d1 = pd[(size_t)(3 * i) * stride];
+    d1 = base_desc(p);
+    set_operand(d1.x, f, B.a32, B.ap);
+    set_operand(d1.y, f, B.a32, B.ap);
+    d1.mode = EPI_STORE;
+    d1.c32 = f == FMT_F32 ? B.p2 : nullptr;
+    d1.out_fmt = f;
+    for (int k = 0; k < 3; ++k) d1.outp[k] = B.p2p[k];
+    ProdDesc& d2 = pd[(size_t)(3 * i + 1) * stride];
+    d2 = base_desc(p);
+    set_operand(d2.x, f, B.p2, B.p2p);
+    set_operand(d2.y, f, B.a32, B.ap);
+    d2.mode = EPI_STORE;
+    d2.c32 = B.p3;
+    d2.out_fmt = f;
+    for (int k = 0; k < 3; ++k) d2.outp[k] = B.p3p[k];
+    ProdDesc& d3 = pd[(size_t)(3 * i + 2) * stride];
+    d3 = base_desc(p);
+    set_operand(d3.x, f, B.p3, B.p3p);
+    set_operand(d3.y, f, B.p2, B.p2p);
+    d3.mode = last ? EPI_POLY_HALF_ID : EPI_POLY;
+    d3.a = sch.a[i];
+    d3.b = sch.b[i];
+    d3.c = sch.c[i];
+    d3.c32 = B.a32;
+    d3.p3 = B.p3;
+    d3.out_fmt = fn;
+    for (int k = 0; k < 3; ++k) d3.outp[k] = B.ap[k];
+  }
+  ProdDesc& d4 = pd[(size_t)(3 * sch.nsteps) * stride];
+  d4 = base_desc(p);
+  set_operand(d4.x, sch.final_fmt, B.x32, B.xp);
+  set_operand(d4.y, sch.final_fmt, B.a32, B.ap);
+  d4.mode = EPI_OUTPUT;
+  d4.out = Y;
+  d4.out_ld = q.n;
+  d4.out_f64 = 0;
+  d4.scale = &st->scale;
+  d4.nonfinite = &st->out_nonfinite;
+}
+
 struct BatchPlan {
   Schedule sch;
   int iters = 20, tight = 0;
@@ -1365,44 +1504,12 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
     if (!blocks[b].X || !blocks[b].Y || blocks[b].n <= 0 || blocks[b].n > 65536)
       return fail(PSDP_ERR_INVALID_ARG, "block %d: bad pointer or size", b);
   bp.count = count;
-  bool any_f32 = false;
-  int pa = pieces_of(sch.final_fmt), pp = 0;
-  for (int i = 0; i < sch.nsteps; ++i) {
-    any_f32 |= sch.fmt[i] == FMT_F32;
-    pa = std::max(pa, pieces_of(sch.fmt[i]));
-    pp = std::max(pp, pieces_of(sch.fmt[i]));
-  }
-  const int px = pieces_of(sch.final_fmt);
-
-  struct BlockBufs {
-    int n, npad, iters;
-    size_t a32, p3, p2, x32, ap[3], p2p[3], p3p[3], xp[3], vec;
-  };
   std::vector<BlockBufs> bb(count);
   size_t off = 0;
   for (int b = 0; b < count; ++b) {
-    BlockBufs& q = bb[b];
-    // kNone marks buffers this schedule does not use (offset 0 is a real buffer)
-    q.p2 = q.x32 = kNone;
-    for (int i = 0; i < 3; ++i) q.ap[i] = q.p2p[i] = q.p3p[i] = q.xp[i] = kNone;
-    q.n = (int)blocks[b].n;
-    const int bt = q.n >= big_n ? 128 : 64;
-    q.npad = (q.n + bt - 1) / bt * bt;
-    q.iters = std::min(bp.iters, q.n);
-    bp.max_npad = std::max(bp.max_npad, q.npad);
-    const int64_t N = (int64_t)q.npad * q.npad;
-    bp.max_N = std::max(bp.max_N, N);
-    const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
-    q.a32 = off, off += f32b;
-    q.p3 = off, off += f32b;
-    if (any_f32) q.p2 = off, off += f32b;
-    if (sch.final_fmt == FMT_F32) q.x32 = off, off += f32b;
-    for (int i = 0; i < pa; ++i) q.ap[i] = off, off += u16b;
-    for (int i = 0; i < pp; ++i) q.p2p[i] = off, off += u16b;
-    for (int i = 0; i < pp; ++i) q.p3p[i] = off, off += u16b;
-    for (int i = 0; i < px; ++i) q.xp[i] = off, off += u16b;
-    q.vec = off;
-    off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
+    off = layout_block(sch, (int)blocks[b].n, bp.iters, big_n, off, bb[b]);
+    bp.max_npad = std::max(bp.max_npad, bb[b].npad);
+    bp.max_N = std::max(bp.max_N, (int64_t)bb[b].npad * bb[b].npad);
   }
   // tiles of each class in XCD-aware order (batched_tile_order)
   std::vector<int> big_blocks, small_blocks;
@@ -1466,96 +1573,8 @@ static psdp_status_t plan_batched(const psdp_block_t* blocks, int count, const p
   memcpy(blob + bp.d_rows, rows.data(), sizeof(int2) * rows.size());
   memcpy(blob + bp.d_chunks, chunks.data(), sizeof(int2) * chunks.size());
 
-  auto F = [&](size_t o) { return o != kNone ? (float*)(base + o) : (float*)nullptr; };
-  auto U = [&](size_t o) { return o != kNone ? (uint16_t*)(base + o) : (uint16_t*)nullptr; };
-  for (int b = 0; b < count; ++b) {
-    const BlockBufs& q = bb[b];
-    Bufs B;
-    B.a32 = F(q.a32);
-    B.p3 = F(q.p3);
-    B.p2 = F(q.p2);
-    B.x32 = F(q.x32);
-    for (int i = 0; i < 3; ++i) {
-      B.ap[i] = U(q.ap[i]);
-      B.p2p[i] = U(q.p2p[i]);
-      B.p3p[i] = U(q.p3p[i]);
-      B.xp[i] = U(q.xp[i]);
-    }
-    const size_t vb = align256((size_t)q.npad * 8);
-    char* v = base + q.vec;
-    LanczosJob& J = lz[b];
-    J.A = B.a32;
-    J.ld = q.npad;
-    J.n = q.n;
-    J.m = q.npad;
-    J.max_iter = q.iters;
-    J.st = states + b;
-    J.v.q = (double*)(v + 0 * vb);
-    J.v.w = (double*)(v + 1 * vb);
-    J.v.w1 = (double*)(v + 2 * vb);
-    J.v.vold = (double*)(v + 3 * vb);
-    J.v.y = (double*)(v + 4 * vb);
-    J.v.ay = (double*)(v + 5 * vb);
-    J.v.V = (double*)(v + 6 * vb);
-    J.v.part = (double*)B.p3;  // GEMV partial sums; A^3 is written only after the bound
-    pj[b] = PadJob{blocks[b].X, q.n, q.npad, B.a32};
-    SplitJob& S = sj[b];
-    S.a32 = B.a32;
-    S.count = (int64_t)q.npad * q.npad;
-    S.inv = &states[b].inv_scale;
-    S.x32 = B.x32;
-    for (int i = 0; i < 3; ++i) {
-      S.ap[i] = B.ap[i];
-      S.xp[i] = B.xp[i];
-    }
-    Plan p;
-    p.n = q.n;
-    p.npad = q.npad;
-    p.ld = q.npad;  // blocks are small: unpadded rows
-    for (int i = 0; i < sch.nsteps; ++i) {
-      const int f = sch.fmt[i];
-      const int fn = i + 1 < sch.nsteps ? sch.fmt[i + 1] : sch.final_fmt;
-      const bool last = i + 1 == sch.nsteps;
-      ProdDesc& d1 = pd[(size_t)(3 * i) * count + b];
-      d1 = base_desc(p);
-      set_operand(d1.x, f, B.a32, B.ap);
-      set_operand(d1.y, f, B.a32, B.ap);
-      d1.mode = EPI_STORE;
-      d1.c32 = f == FMT_F32 ? B.p2 : nullptr;
-      d1.out_fmt = f;
-      for (int k = 0; k < 3; ++k) d1.outp[k] = B.p2p[k];
-      ProdDesc& d2 = pd[(size_t)(3 * i + 1) * count + b];
-      d2 = base_desc(p);
-      set_operand(d2.x, f, B.p2, B.p2p);
-      set_operand(d2.y, f, B.a32, B.ap);
-      d2.mode = EPI_STORE;
-      d2.c32 = B.p3;
-      d2.out_fmt = f;
-      for (int k = 0; k < 3; ++k) d2.outp[k] = B.p3p[k];
-      ProdDesc& d3 = pd[(size_t)(3 * i + 2) * count + b];
-      d3 = base_desc(p);
-      set_operand(d3.x, f, B.p3, B.p3p);
-      set_operand(d3.y, f, B.p2, B.p2p);
-      d3.mode = last ? EPI_POLY_HALF_ID : EPI_POLY;
-      d3.a = sch.a[i];
-      d3.b = sch.b[i];
-      d3.c = sch.c[i];
-      d3.c32 = B.a32;
-      d3.p3 = B.p3;
-      d3.out_fmt = fn;
-      for (int k = 0; k < 3; ++k) d3.outp[k] = B.ap[k];
-    }
-    ProdDesc& d4 = pd[(size_t)(3 * sch.nsteps) * count + b];
-    d4 = base_desc(p);
-    set_operand(d4.x, sch.final_fmt, B.x32, B.xp);
-    set_operand(d4.y, sch.final_fmt, B.a32, B.ap);
-    d4.mode = EPI_OUTPUT;
-    d4.out = blocks[b].Y;
-    d4.out_ld = q.n;
-    d4.out_f64 = 0;
-    d4.scale = &states[b].scale;
-    d4.nonfinite = &states[b].out_nonfinite;
-  }
+  for (int b = 0; b < count; ++b)
+    block_descs(sch, bb[b], base, blocks[b].X, blocks[b].Y, states + b, pd + b, (size_t)count, lz[b], pj[b], sj[b]);
   for (int st = 0; st < bp.nstages; ++st) {
     const int f = st >= 3 * sch.nsteps ? sch.final_fmt : sch.fmt[st / 3];
     for (int b = 0; b < count; ++b) DESCCHK(pd[(size_t)st * count + b], f);
@@ -1729,6 +1748,465 @@ psdp_status_t psdp_project_batched_f32(psdp_handle_t h, const psdp_block_t* bloc
   }
   return PSDP_OK;
 }
+
+// ---------------------------------------------------------------- session (continuous batching)
+// A changing set of blocks advanced one product per tick.  Each block owns a
+// region of the session's fixed arena (buffers as layout_block places them,
+// then its Lanczos state) and its host-side launch descriptors
+// (block_descs).  A block goes through
+//   * its prologue -- symmetrize/pad, the batched Lanczos bound and the
+//     scale/split, the kernels of the batched call -- run for all blocks
+//     added since the last prologue on the handle's second stream, so that
+//     the latency-bound Lanczos launches overlap the products;
+//   * once the host has seen that prologue's event complete, one product
+//     per tick: a tick is, on the handle's stream, one persistent grouped
+//     launch per (operand format, averaged, tile class) over the next
+//     product of every ready block -- the same kernels and descriptors as
+//     the batched call, so a block's result is bitwise the one it gets alone.
+// Descriptors and work lists go up in one copy per prologue / tick from the
+// handle's pinned staging ring into the session's device areas (one per
+// stream; stream order keeps each copy behind the kernels that read the
+// previous one).  A block's region is reused only after the event of the
+// tick that ran its output product has completed.
+struct SessBlock {
+  int64_t ticket = 0;
+  size_t off = 0, bytes = 0;  // arena region
+  BlockBufs q;
+  LanczosState* st = nullptr;
+  std::vector<ProdDesc> pd;   // one per stage
+  LanczosJob lz;
+  PadJob pj;
+  SplitJob sj;
+  int stage = -2;             // -2: prologue not enqueued, -1: prologue running, then the next product
+};
+
+struct DevArea {
+  char* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct psdp_session_st {
+  psdp_handle_t h = nullptr;
+  Schedule sch;
+  int iters = 20, tight = 0, big_n = BIG_BLOCK_N, nstages = 0;
+  double tol = 1e-10, margin = 1.0;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  DevArea tick_area, pro_area;
+  hipStream_t pro_stream = nullptr;  // prologues
+  struct Prologue {
+    hipEvent_t ev;
+    std::vector<int64_t> tickets;
+  };
+  std::deque<Prologue> pros;         // enqueued, not yet seen complete (in stream order)
+  std::map<size_t, size_t> free_;  // offset -> size
+  std::vector<SessBlock> blocks;   // admitted, not finished
+  struct Done {
+    int64_t ticket;
+    size_t off, bytes;
+  };
+  struct Tick {
+    hipEvent_t ev;
+    std::vector<Done> finished;
+  };
+  std::deque<Tick> ticks;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int64_t> completed;
+  int64_t next_ticket = 1;
+};
+
+// Arena bytes of a block of size n: its buffers, then its Lanczos state.
+static size_t session_block_bytes(const psdp_session_st* s, int n, size_t* state_off) {
+  BlockBufs q;
+  const size_t end = layout_block(s->sch, n, s->iters, s->big_n, 0, q);
+  *state_off = align256(end);
+  return *state_off + align256(sizeof(LanczosState));
+}
+
+// First fit over the free regions (offset -> size, coalesced on free).
+static bool session_alloc(psdp_session_st* s, size_t bytes, size_t* off) {
+  for (auto it = s->free_.begin(); it != s->free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    *off = it->first;
+    const size_t rest = it->second - bytes;
+    const size_t at = it->first + bytes;
+    s->free_.erase(it);
+    if (rest) s->free_[at] = rest;
+    return true;
+  }
+  return false;
+}
+
+static void session_free(psdp_session_st* s, size_t off, size_t bytes) {
+  auto it = s->free_.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != s->free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    s->free_.erase(nx);
+  }
+  if (it != s->free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      s->free_.erase(it);
+    }
+  }
+}
+
+// true when the event has completed (false: not yet); errors become a status
+static psdp_status_t event_done(hipEvent_t e, bool* done) {
+  const hipError_t q = hipEventQuery(e);
+  if (q == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = false;
+    return PSDP_OK;
+  }
+  HIPCHK(q);
+  *done = true;
+  return PSDP_OK;
+}
+
+// Retires completed ticks, oldest first (their finished blocks give back
+// their regions and report their tickets), and makes the blocks of a
+// completed prologue ready for ticks.  Caller holds h->mu.
+static psdp_status_t session_retire(psdp_session_st* s) {
+  while (!s->ticks.empty()) {
+    bool d = false;
+    STCHK(event_done(s->ticks.front().ev, &d));
+    if (!d) break;
+    auto& t = s->ticks.front();
+    for (const auto& f : t.finished) {
+      session_free(s, f.off, f.bytes);
+      s->completed.push_back(f.ticket);
+    }
+    s->ev_pool.push_back(t.ev);
+    s->ticks.pop_front();
+  }
+  while (!s->pros.empty()) {
+    bool d = false;
+    STCHK(event_done(s->pros.front().ev, &d));
+    if (!d) break;
+    const auto& tk = s->pros.front().tickets;
+    for (auto& b : s->blocks)
+      if (b.stage == -1 && std::find(tk.begin(), tk.end(), b.ticket) != tk.end()) b.stage = 0;
+    s->ev_pool.push_back(s->pros.front().ev);
+    s->pros.pop_front();
+  }
+  return PSDP_OK;
+}
+
+extern "C++" {
+// Appends a typed array to a host blob (256-byte aligned); returns its offset.
+template <typename T>
+static size_t blob_put(std::vector<char>& blob, const T* data, size_t count) {
+  const size_t off = align256(blob.size());
+  blob.resize(off + align256(sizeof(T) * std::max<size_t>(count, 1)), 0);
+  if (count) memcpy(blob.data() + off, data, sizeof(T) * count);
+  return off;
+}
+}
+
+// Copies a host blob to a session device area on `st` (through the handle's
+// staging ring); grows the area first if needed.
+static psdp_status_t session_upload(psdp_session_st* s, DevArea& a, const std::vector<char>& blob, hipStream_t st) {
+  psdp_handle_t h = s->h;
+  if (blob.size() > a.bytes) {
+    HIPCHK(hipStreamSynchronize(st));
+    if (a.p) HIPCHK(hipFree(a.p));
+    a.p = nullptr;
+    a.bytes = 0;
+    const size_t want = std::max<size_t>(blob.size() * 2, 1 << 20);
+    if (hipMalloc(&a.p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PSDP_ERR_NOMEM, "session descriptor area of %zu bytes", want);
+    }
+    a.bytes = want;
+  }
+  char* stage = nullptr;
+  STCHK(staging_slot(h, blob.size(), &stage));
+  memcpy(stage, blob.data(), blob.size());
+  HIPCHK(hipMemcpyAsync(a.p, stage, blob.size(), hipMemcpyHostToDevice, st));
+  STCHK(staging_release(h, st));
+  return PSDP_OK;
+}
+
+// Enqueues the prologue of every block not yet started on the prologue
+// stream (stream order keeps each upload behind the kernels of the previous
+// prologue, so several may be queued).  Caller holds h->mu.
+static psdp_status_t session_prologue(psdp_session_st* s) {
+  psdp_handle_t h = s->h;
+  std::vector<int> fresh;
+  for (int i = 0; i < (int)s->blocks.size(); ++i)
+    if (s->blocks[i].stage == -2) fresh.push_back(i);
+  if (fresh.empty() || s->pros.size() >= 4) return PSDP_OK;
+  std::vector<LanczosJob> lz;
+  std::vector<PadJob> pj;
+  std::vector<SplitJob> sj;
+  std::vector<TileRef> padt, symt;
+  std::vector<int2> rows, chunks;
+  int max_m = 0;
+  for (int g = 0; g < (int)fresh.size(); ++g) {
+    const SessBlock& b = s->blocks[fresh[g]];
+    lz.push_back(b.lz);
+    pj.push_back(b.pj);
+    sj.push_back(b.sj);
+    const int T = b.q.npad / 64;
+    for (int i = 0; i < T; ++i)
+      for (int j = i; j < T; ++j) padt.push_back(TileRef{g, i, j, 0});
+    const int nt = (b.q.npad + SYMV_TILE - 1) / SYMV_TILE;
+    for (int j = 0; j < nt; ++j)
+      for (int i = 0; i <= j; ++i) symt.push_back(TileRef{g, i, j, 0});
+    for (int r = 0; r < nt; ++r) rows.push_back(make_int2(g, r));
+    const int64_t nc = (int64_t)b.q.npad * b.q.npad / SPLIT_CHUNK;
+    for (int64_t c = 0; c < nc; ++c) chunks.push_back(make_int2(g, (int)c));
+    max_m = std::max(max_m, b.q.npad);
+  }
+  std::vector<char> blob;
+  const size_t o_lz = blob_put(blob, lz.data(), lz.size());
+  const size_t o_pj = blob_put(blob, pj.data(), pj.size());
+  const size_t o_sj = blob_put(blob, sj.data(), sj.size());
+  const size_t o_padt = blob_put(blob, padt.data(), padt.size());
+  const size_t o_symt = blob_put(blob, symt.data(), symt.size());
+  const size_t o_rows = blob_put(blob, rows.data(), rows.size());
+  const size_t o_chunks = blob_put(blob, chunks.data(), chunks.size());
+  // A region handed back by an earlier block is reused only after the host
+  // saw that block's last tick complete, so no wait on the handle's stream.
+  hipStream_t ps = s->pro_stream;
+  STCHK(session_upload(s, s->pro_area, blob, ps));
+  char* D = s->pro_area.p;
+  HIPCHK(launch_sym_pad_batched((const PadJob*)(D + o_pj), (const TileRef*)(D + o_padt), (int)padt.size(), ps));
+  HIPCHK(launch_lanczos_batched((const LanczosJob*)(D + o_lz), (int)fresh.size(), max_m, s->iters, s->tol, h->seed,
+                                s->margin, s->tight, (const TileRef*)(D + o_symt), (int)symt.size(),
+                                (const int2*)(D + o_rows), (int)rows.size(), ps));
+  HIPCHK(launch_scale_split_batched((const SplitJob*)(D + o_sj), (const int2*)(D + o_chunks), (int)chunks.size(),
+                                    s->sch.fmt[0], s->sch.final_fmt, ps));
+  psdp_session_st::Prologue pr;
+  if (!s->ev_pool.empty()) {
+    pr.ev = s->ev_pool.back();
+    s->ev_pool.pop_back();
+  } else {
+    HIPCHK(hipEventCreateWithFlags(&pr.ev, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(pr.ev, ps));
+  for (int i : fresh) {
+    s->blocks[i].stage = -1;
+    pr.tickets.push_back(s->blocks[i].ticket);
+  }
+  s->pros.push_back(std::move(pr));
+  return PSDP_OK;
+}
+
+// Enqueues one tick over the ready blocks (stage >= 0).  Caller holds h->mu.
+static psdp_status_t session_tick(psdp_session_st* s) {
+  psdp_handle_t h = s->h;
+  hipStream_t st = h->stream;
+  struct Group {
+    int fmt, avg, sub;
+    std::vector<int> members;  // indices into s->blocks
+    size_t o_desc = 0, o_tiles = 0;
+    int ntiles = 0;
+  };
+  std::vector<Group> groups;
+  for (int i = 0; i < (int)s->blocks.size(); ++i) {
+    const SessBlock& b = s->blocks[i];
+    if (b.stage < 0) continue;
+    const int stg = b.stage;
+    const bool fin = stg >= 3 * s->sch.nsteps;
+    const int f = fin ? s->sch.final_fmt : s->sch.fmt[stg / 3];
+    const int avg = fin ? s->sch.final_avg : (stg % 3 != 0 ? s->sch.avg[stg / 3] : 0);
+    const int sub = block_tile(b.q.n, s->big_n) / 32;
+    auto it = std::find_if(groups.begin(), groups.end(),
+                           [&](const Group& g) { return g.fmt == f && g.avg == avg && g.sub == sub; });
+    if (it == groups.end()) {
+      groups.push_back(Group{f, avg, sub, {}});
+      it = groups.end() - 1;
+    }
+    it->members.push_back(i);
+  }
+  if (groups.empty()) return PSDP_OK;
+  std::vector<char> blob;
+  for (auto& g : groups) {
+    std::vector<ProdDesc> d;
+    std::vector<int> npads, local;
+    for (int k = 0; k < (int)g.members.size(); ++k) {
+      const SessBlock& b = s->blocks[g.members[k]];
+      d.push_back(b.pd[b.stage]);
+      DESCCHK(d.back(), g.fmt);
+      npads.push_back(b.q.npad);
+      local.push_back(k);
+    }
+    const std::vector<TileRef> tl = batched_tile_order(local, 32 * g.sub, npads);
+    g.o_desc = blob_put(blob, d.data(), d.size());
+    g.o_tiles = blob_put(blob, tl.data(), tl.size());
+    g.ntiles = (int)tl.size();
+  }
+  STCHK(session_upload(s, s->tick_area, blob, st));
+  char* D = s->tick_area.p;
+  for (const auto& g : groups) {
+    if (!g.ntiles) continue;
+    ProfScope pg(h, PSDP_PROF_PROD_F32 + g.fmt);
+    const int grid = std::min(g.ntiles, g.sub >= 4 ? h->cus : 4 * h->cus);
+    HIPCHK(launch_sym_product_grouped(g.fmt, g.sub, g.avg, (const ProdDesc*)(D + g.o_desc),
+                                      (const TileRef*)(D + g.o_tiles), g.ntiles, grid, st));
+  }
+  psdp_session_st::Tick t;
+  if (!s->ev_pool.empty()) {
+    t.ev = s->ev_pool.back();
+    s->ev_pool.pop_back();
+  } else {
+    HIPCHK(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
+  }
+  std::vector<SessBlock> keep;
+  keep.reserve(s->blocks.size());
+  for (auto& b : s->blocks) {
+    if (b.stage >= 0 && ++b.stage >= s->nstages) t.finished.push_back(psdp_session_st::Done{b.ticket, b.off, b.bytes});
+    else keep.push_back(std::move(b));
+  }
+  s->blocks.swap(keep);
+  HIPCHK(hipEventRecord(t.ev, st));
+  s->ticks.push_back(std::move(t));
+  return PSDP_OK;
+}
+
+extern "C" {
+
+psdp_status_t psdp_session_create(psdp_handle_t h, const psdp_config_t* cfg, size_t arena_bytes,
+                                  psdp_session_t* out) {
+  if (!h || !out) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  psdp_config_t def;
+  if (!cfg) {
+    psdp_default_config(&def, PSDP_METHOD_MIXED);
+    cfg = &def;
+  }
+  if (!cfg->auto_scale) return fail(PSDP_ERR_UNSUPPORTED, "sessions project with the Lanczos bound (auto_scale = 1)");
+  if (arena_bytes < (1 << 20)) return fail(PSDP_ERR_INVALID_ARG, "session arena below 1 MiB");
+  auto* s = new psdp_session_st();
+  s->h = h;
+  const psdp_status_t sst = build_schedule(cfg, s->sch);
+  if (sst != PSDP_OK) {
+    delete s;
+    return sst;
+  }
+  s->iters = cfg->lanczos_iters > 0 ? cfg->lanczos_iters : 20;
+  if (s->iters > LANCZOS_MAX_ITER) {
+    delete s;
+    return fail(PSDP_ERR_INVALID_ARG, "at most %d Lanczos iterations are supported", LANCZOS_MAX_ITER);
+  }
+  s->tol = cfg->lanczos_tol > 0 ? cfg->lanczos_tol : 1e-10;
+  s->margin = cfg->scale_margin > 0 ? cfg->scale_margin : 1.0;
+  s->tight = cfg->bound_mode == PSDP_BOUND_TIGHT;
+  s->nstages = 3 * s->sch.nsteps + 1;
+  {
+    std::lock_guard<std::mutex> lock(h->mu);
+    s->big_n = h->batch_big_n > 0 ? h->batch_big_n : BIG_BLOCK_N;
+    if (hipSetDevice(h->device) != hipSuccess || hipMalloc(&s->arena, arena_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      delete s;
+      return fail(PSDP_ERR_NOMEM, "session arena of %zu bytes", arena_bytes);
+    }
+  }
+  s->arena_bytes = arena_bytes;
+  s->free_[0] = arena_bytes;
+  if (hipStreamCreateWithFlags(&s->pro_stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    if (s->pro_stream) (void)hipStreamDestroy(s->pro_stream);
+    (void)hipFree(s->arena);
+    delete s;
+    return fail(PSDP_ERR_DEVICE, "session stream creation failed");
+  }
+  *out = s;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_session_destroy(psdp_session_t s) {
+  if (!s) return PSDP_OK;
+  psdp_handle_t h = s->h;
+  std::lock_guard<std::mutex> lock(h->mu);
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  if (s->pro_stream) (void)hipStreamSynchronize(s->pro_stream);
+  for (auto& t : s->ticks) (void)hipEventDestroy(t.ev);
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  for (auto& p : s->pros) (void)hipEventDestroy(p.ev);
+  if (s->pro_stream) (void)hipStreamDestroy(s->pro_stream);
+  if (s->arena) (void)hipFree(s->arena);
+  if (s->tick_area.p) (void)hipFree(s->tick_area.p);
+  if (s->pro_area.p) (void)hipFree(s->pro_area.p);
+  delete s;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_session_block_bytes(psdp_session_t s, int64_t n, size_t* bytes) {
+  if (!s || !bytes) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad block size %lld", (long long)n);
+  size_t so = 0;
+  *bytes = session_block_bytes(s, (int)n, &so);
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_session_add(psdp_session_t s, const float* X, float* Y, int64_t n, int64_t* ticket) {
+  if (!s || !X || !Y || !ticket) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  if (n <= 0 || n > 65536) return fail(PSDP_ERR_INVALID_ARG, "bad block size %lld", (long long)n);
+  psdp_handle_t h = s->h;
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  STCHK(session_retire(s));
+  size_t so = 0;
+  const size_t bytes = session_block_bytes(s, (int)n, &so);
+  if (bytes > s->arena_bytes)
+    return fail(PSDP_ERR_INVALID_ARG, "a block of n = %lld needs %zu bytes, more than the session arena (%zu)",
+                (long long)n, bytes, s->arena_bytes);
+  size_t off = 0;
+  if (!session_alloc(s, bytes, &off))
+    return fail(PSDP_ERR_NOMEM, "no room in the session arena for n = %lld now (%zu bytes)", (long long)n, bytes);
+  SessBlock b;
+  b.ticket = s->next_ticket++;
+  b.off = off;
+  b.bytes = bytes;
+  layout_block(s->sch, (int)n, s->iters, s->big_n, off, b.q);
+  b.st = (LanczosState*)(s->arena + off + so);
+  b.pd.resize(s->nstages);
+  block_descs(s->sch, b.q, s->arena, X, Y, b.st, b.pd.data(), 1, b.lz, b.pj, b.sj);
+  b.stage = -2;
+  *ticket = b.ticket;
+  s->blocks.push_back(std::move(b));
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_session_step(psdp_session_t s, int* in_flight_ticks) {
+  if (!s) return fail(PSDP_ERR_INVALID_ARG, "null session");
+  psdp_handle_t h = s->h;
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  STCHK(session_retire(s));
+  STCHK(session_prologue(s));
+  STCHK(session_tick(s));
+  if (in_flight_ticks) *in_flight_ticks = (int)s->ticks.size();
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_session_poll(psdp_session_t s, int64_t* tickets, int max, int* count, int* active,
+                                int* in_flight_ticks) {
+  if (!s || !count || (max > 0 && !tickets)) return fail(PSDP_ERR_INVALID_ARG, "null argument");
+  psdp_handle_t h = s->h;
+  std::lock_guard<std::mutex> lock(h->mu);
+  HIPCHK(hipSetDevice(h->device));
+  STCHK(session_retire(s));
+  const int k = std::min<int>(max, (int)s->completed.size());
+  for (int i = 0; i < k; ++i) tickets[i] = s->completed[i];
+  s->completed.erase(s->completed.begin(), s->completed.begin() + k);
+  *count = k;
+  if (active) {
+    int a = (int)s->blocks.size();
+    for (const auto& t : s->ticks) a += (int)t.finished.size();
+    *active = a;
+  }
+  if (in_flight_ticks) *in_flight_ticks = (int)s->ticks.size();
+  return PSDP_OK;
+}
+
+}  // extern "C"
 
 psdp_status_t psdp_project_packed_f32(psdp_handle_t h, const float* X, float* Y, const int64_t* sizes, int count,
                                       const psdp_config_t* cfg) {

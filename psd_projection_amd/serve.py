@@ -1,21 +1,31 @@
 """Served projection stream: continuous batching of mixed-size requests.
 
 An ADMM/SDP solver hands the server symmetric float32 matrices that already
-live on the device and collects their projections.  Requests are admitted in
-arrival order while their memory fits an HBM budget; admitted requests run on
-several lanes (HIP streams) at once, so work that arrives while other work is
-on the GPU starts as soon as a lane and memory free up:
+live on the device and collects their projections (the reference's intended
+caller, include/psd_projection/check.h:10, projects cone blocks every
+iteration).  Requests are admitted in arrival order while their memory fits
+an HBM budget and run at once:
 
-* small lanes (n <= small_max): every launch takes all admitted small
-  requests at once and projects them with one grouped launch per filter-chain
-  product (psdp_project_batched_f32), so requests that queued while the
-  previous batch ran form the next batch;
-* large lanes: one request per launch (psdp_project_f32), one handle and
-  stream per lane, so a large projection never blocks the small ones.
+* small requests (n <= small_max) join a session (psdp_session_*): every
+  tick runs ONE filter-chain product of every request in it as one grouped
+  launch per operand format, so a request that arrives while others are on
+  the GPU starts at the next product boundary (after its own Lanczos bound,
+  which runs on a second stream beside the products) instead of waiting for
+  the running batch to finish, and leaves after its own output product;
+* large requests run one per lane (psdp_project_f32, one handle and stream
+  per lane), so a large projection never blocks the small ones.
 
-A request's result does not depend on what else is in flight: the grouped
-path gives a block bitwise the same result alone and in any batch, and the
-single path runs the same kernels on its own stream.
+A request's result does not depend on what else is in flight: a session
+block gets bitwise the result psdp_project_batched_f32 gives it alone, and a
+lane runs the single-matrix kernels on its own stream.
+
+Memory: the budget covers what the executor holds for the whole run (the
+session's arena and the lanes' workspaces, sized once at warm-up) plus the
+outputs of admitted requests until the caller releases them (release(), done
+automatically after on_done returns).  A small request that does not fit the
+session's arena yet waits in the queue; a device allocation that fails at
+admission (torch's OutOfMemoryError) requeues the request instead of
+crashing the loop.
 
 The executor is pluggable (``TorchExecutor`` on the GPU; tests drive the
 scheduler with a fake one on the CPU).
@@ -25,7 +35,7 @@ from __future__ import annotations
 import collections
 import dataclasses
 import time
-from typing import Any, Callable, Deque, List, Optional
+from typing import Any, Callable, Deque, Dict, List, Optional
 
 
 @dataclasses.dataclass
@@ -35,11 +45,12 @@ class Request:
     X: Any                      # device tensor (n x n, float32), read only
     t_arrival: float            # host seconds (time.perf_counter clock)
     Y: Any = None
-    bytes: int = 0
-    lane: int = -1
+    bytes: int = 0              # output bytes charged to the budget until released
+    lane: int = -1              # large lane, or -1 for the session
     t_launch: float = 0.0
     t_done: float = 0.0
     err: Any = None             # optional device scalar (relative error), set by a callback
+    released: bool = False
 
     @property
     def latency(self) -> float:
@@ -47,153 +58,228 @@ class Request:
 
 
 class TorchExecutor:
-    """Runs lanes on HIP streams through libpsdp (GPU)."""
+    """Session (small requests) and lanes (large requests) on HIP streams
+    through libpsdp (GPU)."""
 
-    # One small lane: whatever arrives while a batch runs forms the next batch.
-    # Measured on the 10,000-request stream at 0.7 x capacity: 1 small + 2
-    # large lanes gave p50 9.2 ms / p99 110 ms (small requests' p99 34 ms),
-    # 2 + 2 gave p50 20 ms / p99 109 ms (small requests' p99 77 ms) at 3.5%
-    # more capacity.
-    def __init__(self, device=0, method="mixed", small_lanes=1, large_lanes=2, seed=5):
+    # One large lane by default.  Measured on the 5-size stream at 390/s: with
+    # two lanes side by side the large products hold every CU more of the time
+    # and the session's short kernels wait for them (small requests' p50 30 ms,
+    # p99 285 ms, burst capacity 643/s); with one lane 9.5 / 45 ms at 594/s.
+    # Without large requests in the stream the session alone gives 1.1 / 3.3 ms.
+    def __init__(self, device=0, method="mixed", large_lanes=1, seed=5, session_bytes=6 << 30,
+                 small_max=1024, ticks_ahead=4):
         import torch
-        from . import Handle, default_config, workspace_size, workspace_size_batched
+        from . import Handle, Session, default_config, workspace_size
         self.torch = torch
         self.dev = torch.device("cuda", device)
         self.cfg = default_config(method, synchronize=0)
         self.method = method
+        self.small_max = small_max
+        self.ticks_ahead = ticks_ahead
         self._ws = workspace_size
-        self._wsb = workspace_size_batched
-        self.small = small_lanes
+        self.sess_stream = torch.cuda.Stream(device=self.dev)
+        self.sess_handle = Handle(device, stream=self.sess_stream, seed=seed)
+        self.session = Session(self.sess_handle, session_bytes, config=self.cfg)
+        self.session_bytes = int(session_bytes)
+        self.tickets: Dict[int, Request] = {}
         self.lanes = []
-        for i in range(small_lanes + large_lanes):
+        for _ in range(large_lanes):
             s = torch.cuda.Stream(device=self.dev)
-            h = Handle(device, stream=s, seed=seed)
-            self.lanes.append((s, h))
+            self.lanes.append((s, Handle(device, stream=s, seed=seed)))
+        self.lane_bytes = 0
 
-    def is_small_lane(self, lane: int) -> bool:
-        return lane < self.small
+    # ------------------------------------------------------------------ memory
+    def fixed_bytes(self) -> int:
+        return self.session_bytes + self.lane_bytes
 
-    def request_bytes(self, n: int, small: bool) -> int:
-        io = 4 * n * n
-        ws = self._wsb([n], self.method) if small else self._ws(n, self.method)
-        return io + ws
+    def output_bytes(self, n: int) -> int:
+        return 4 * n * n
 
-    def launch(self, lane: int, reqs: List[Request]):
-        from . import project, project_batched
+    def fits_session(self, n: int) -> bool:
+        return self.session.block_bytes(n) <= self.session_bytes
+
+    def _alloc_out(self, r: Request) -> bool:
         torch = self.torch
+        try:
+            r.Y = torch.empty((r.n, r.n), dtype=torch.float32, device=self.dev)
+            return True
+        except torch.OutOfMemoryError:
+            r.Y = None
+            return False
+
+    # ------------------------------------------------------------------ session
+    def session_add(self, r: Request) -> bool:
+        """Admits r into the session; False when the arena (or the output
+        allocation) has no room now."""
+        if not self._alloc_out(r):
+            return False
+        t = self.session.add(r.X, r.Y)
+        if t is None:
+            r.Y = None
+            return False
+        self.tickets[t] = r
+        return True
+
+    def session_step(self) -> List[Request]:
+        """Returns the requests whose outputs completed; enqueues the next tick
+        (and the prologue of new requests) while fewer than ticks_ahead are
+        in flight, so that a request added now joins within that many ticks."""
+        done, active, inflight = self.session.poll()
+        out = [self.tickets.pop(t) for t in done]
+        if active and inflight < self.ticks_ahead:
+            self.session.step()
+        return out
+
+    def session_active(self) -> bool:
+        return bool(self.tickets)
+
+    # ------------------------------------------------------------------ lanes
+    def launch(self, lane: int, r: Request):
+        torch = self.torch
+        from . import project
         s, h = self.lanes[lane]
         with torch.cuda.stream(s):
-            for r in reqs:
-                r.Y = torch.empty((r.n, r.n), dtype=torch.float32, device=self.dev)
-            if self.is_small_lane(lane):
-                project_batched(h, [r.X for r in reqs], outs=[r.Y for r in reqs], config=self.cfg)
-            else:
-                for r in reqs:
-                    project(h, r.X, out=r.Y, config=self.cfg)
+            project(h, r.X, out=r.Y, config=self.cfg)
             ev = torch.cuda.Event()
             ev.record(s)
         return ev
 
+    def alloc_large(self, r: Request) -> bool:
+        return self._alloc_out(r)
+
     def done(self, token) -> bool:
         return token.query()
 
-    def stream_of(self, lane: int):
-        return self.lanes[lane][0]
+    def stream_of(self, r: Request):
+        return self.sess_stream if r.lane < 0 else self.lanes[r.lane][0]
 
     def warm(self, sizes):
-        """Tunes the large lanes' tile sizes and sizes the lane arenas and tile
-        lists for the largest requests (keeps both out of the timed loop)."""
+        """Tunes the lanes' tile sizes and sizes their arenas and tile lists for
+        the largest requests (keeps both out of the timed loop; their bytes
+        are then charged to the budget once)."""
         torch = self.torch
-        small = [n for n in sizes if n <= 1024]
-        large = [n for n in sizes if n > 1024]
+        large = sorted({n for n in sizes if n > self.small_max})
         for lane in range(len(self.lanes)):
-            if not self.is_small_lane(lane):
-                for n in large:  # tile sizes of the single-matrix projections
-                    self.lanes[lane][1].autotune(n, config=self.cfg)
-        for lane in range(len(self.lanes)):
-            ns = small if self.is_small_lane(lane) else large
-            if not ns:
-                continue
-            xs = [torch.eye(n, device=self.dev) for n in ns]
-            reqs = [Request(-1, n, x, 0.0) for n, x in zip(ns, xs)]
-            if self.is_small_lane(lane):
-                self.done(self.launch(lane, reqs))
-            else:
-                for r in reqs:
-                    self.launch(lane, [r])
+            s, h = self.lanes[lane]
+            for n in large:
+                h.autotune(n, config=self.cfg)
+                if n >= 4096 and len(self.lanes) > 1:
+                    # the lanes run side by side, so one projection need not
+                    # fill the GPU with its own tiles: 256 x 256 tiles (the
+                    # fastest per tile) gave 659 against 643 projections/s of
+                    # burst capacity on the 5-size stream
+                    for f in ("f16x3", "f16", "bf16x3", "bf16"):
+                        h.set_tile(n, f, 256)
+                x = torch.eye(n, device=self.dev)
+                r = Request(-1, n, x, 0.0, Y=torch.empty_like(x))
+                torch.cuda.synchronize()
+                self.launch(lane, r)
+        if large:
+            self.lane_bytes = len(self.lanes) * max(self._ws(n, self.method) for n in large)
+        small = sorted({n for n in sizes if n <= self.small_max})
+        if small:  # one pass of the session over every small size (prologue and grouped kernels warm)
+            xs = [torch.eye(n, device=self.dev) for n in small]
+            ys = [torch.empty_like(x) for x in xs]
+            torch.cuda.synchronize()
+            pending = len(xs)
+            for x, y in zip(xs, ys):
+                assert self.session.add(x, y) is not None
+            while pending:
+                done, _, _ = self.session.poll()
+                pending -= len(done)
+                self.session.step()
         torch.cuda.synchronize()
 
 
 class ProjectionServer:
-    """Admission by HBM budget + continuous batching over lanes."""
+    """Admission by HBM budget + continuous batching (session + lanes)."""
 
-    def __init__(self, executor, hbm_budget: int, small_max: int = 1024, max_batch: int = 512,
+    def __init__(self, executor, hbm_budget: int, small_max: int = 1024,
                  on_done: Optional[Callable[[Request], None]] = None, clock=time.perf_counter):
         self.ex = executor
         self.budget = int(hbm_budget)
         self.small_max = small_max
-        self.max_batch = max_batch
         self.on_done = on_done
         self.clock = clock
         self.pending: Deque[Request] = collections.deque()     # arrived, not admitted
-        self.admitted_small: Deque[Request] = collections.deque()
         self.admitted_large: Deque[Request] = collections.deque()
-        self.inflight = {}                                       # lane -> (token, [reqs])
-        self.used = 0
+        self.inflight = {}                                       # lane -> (token, req)
+        self.used = 0                                            # outputs held (admitted, not released)
         self.peak_used = 0
         self.completed: List[Request] = []
         self.batches = 0
+        self.refusals = 0
+        if self.ex.fixed_bytes() > self.budget:
+            raise ValueError(f"the executor holds {self.ex.fixed_bytes()} B, more than the budget {self.budget}")
 
     # ------------------------------------------------------------------ api
     def submit(self, req: Request):
-        small = req.n <= self.small_max
-        req.bytes = self.ex.request_bytes(req.n, small)
-        if req.bytes > self.budget:
-            raise ValueError(f"request {req.rid} (n={req.n}) needs {req.bytes} B, more than the budget")
+        req.bytes = self.ex.output_bytes(req.n)
+        if self.ex.fixed_bytes() + req.bytes > self.budget:
+            raise ValueError(f"request {req.rid} (n={req.n}) needs {req.bytes} B, more than the budget leaves")
+        if req.n <= self.small_max and not self.ex.fits_session(req.n):
+            raise ValueError(f"request {req.rid} (n={req.n}) does not fit the session arena")
         self.pending.append(req)
 
+    def release(self, req: Request):
+        """The caller is done with req.Y: its bytes leave the budget."""
+        if not req.released:
+            req.released = True
+            req.Y = None
+            self.used -= req.bytes
+
+    def held_bytes(self) -> int:
+        return self.ex.fixed_bytes() + self.used
+
     def idle(self) -> bool:
-        return not (self.pending or self.admitted_small or self.admitted_large or self.inflight)
+        return not (self.pending or self.admitted_large or self.inflight or self.ex.session_active())
+
+    def _finish(self, r: Request, now: float):
+        r.t_done = now
+        self.completed.append(r)
+        if self.on_done is not None:
+            self.on_done(r)
+            self.release(r)
 
     def step(self):
-        """Retire finished lanes, admit in arrival order, launch idle lanes."""
+        """Retire finished work, admit in arrival order, advance the session,
+        launch idle lanes."""
         now = self.clock()
         for lane in list(self.inflight):
-            token, reqs = self.inflight[lane]
+            token, r = self.inflight[lane]
             if self.ex.done(token):
                 del self.inflight[lane]
-                for r in reqs:
-                    r.t_done = now
-                    self.used -= r.bytes
-                    if self.on_done is not None:
-                        self.on_done(r)
-                    self.completed.append(r)
-        # admission: FIFO while the head fits the budget
-        while self.pending and self.used + self.pending[0].bytes <= self.budget:
-            r = self.pending.popleft()
-            self.used += r.bytes
-            self.peak_used = max(self.peak_used, self.used)
-            (self.admitted_small if r.n <= self.small_max else self.admitted_large).append(r)
-        # launch
-        nlanes = len(self.ex.lanes)
-        for lane in range(nlanes):
-            if lane in self.inflight:
-                continue
-            if self.ex.is_small_lane(lane):
-                if not self.admitted_small:
-                    continue
-                batch = []
-                while self.admitted_small and len(batch) < self.max_batch:
-                    batch.append(self.admitted_small.popleft())
+                self._finish(r, now)
+        # admission: FIFO while the head's output fits the budget (and, for a
+        # small request, the session's arena)
+        while self.pending:
+            r = self.pending[0]
+            if self.held_bytes() + r.bytes > self.budget:
+                break
+            if r.n <= self.small_max:
+                if not self.ex.session_add(r):
+                    self.refusals += 1
+                    break
+                r.t_launch = self.clock()
             else:
-                if not self.admitted_large:
-                    continue
-                batch = [self.admitted_large.popleft()]
-            t = self.clock()
-            for r in batch:
-                r.lane = lane
-                r.t_launch = t
-            self.inflight[lane] = (self.ex.launch(lane, batch), batch)
+                if not self.ex.alloc_large(r):
+                    self.refusals += 1
+                    break
+                self.admitted_large.append(r)
+            self.pending.popleft()
+            self.used += r.bytes
+            self.peak_used = max(self.peak_used, self.held_bytes())
+        # the session: one product of every small request per tick
+        for r in self.ex.session_step():
+            self._finish(r, self.clock())
+        # lanes
+        for lane in range(len(self.ex.lanes)):
+            if lane in self.inflight or not self.admitted_large:
+                continue
+            r = self.admitted_large.popleft()
+            r.lane = lane
+            r.t_launch = self.clock()
+            self.inflight[lane] = (self.ex.launch(lane, r), r)
             self.batches += 1
 
     def run(self, requests: List[Request], arrival_offsets: List[float], poll_sleep: float = 0.0):
