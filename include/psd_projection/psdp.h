@@ -231,10 +231,12 @@ psdp_status_t psdp_project_packed_f32(psdp_handle_t h, const float* X, float* Y,
  * workspace comes from the session's arena, sized once at creation, so
  * nothing is allocated per block: PSDP_ERR_NOMEM says the arena has no room
  * for it now (retry after completions), PSDP_ERR_INVALID_ARG that it never
- * will.  Each psdp_session_step enqueues one tick: the prologue (symmetrize,
- * Lanczos bound, scale/split) of the blocks added since the last tick, then
- * ONE filter-chain product for every block past its prologue, one grouped
- * launch per operand format and tile class.  A block thus starts at the next
+ * will.  Each psdp_session_step enqueues the prologue (symmetrize, Lanczos
+ * bound, scale/split) of the blocks added since the last step on the
+ * session's second stream (at most four prologues in flight), and one tick on
+ * the handle's stream: ONE filter-chain product for every block whose
+ * prologue has completed, one grouped launch per operand format and tile
+ * class.  A block thus starts at the next
  * product boundary instead of waiting for the blocks ahead of it, and leaves
  * after its output product; *in_flight_ticks (may be NULL) counts the ticks
  * enqueued and not yet complete.  psdp_session_poll returns (up to max)
