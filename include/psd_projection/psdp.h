@@ -66,8 +66,12 @@ typedef enum psdp_format {
   PSDP_FMT_BF16 = 3,   /* bfloat16 operands                                        */
   PSDP_FMT_BF16X3 = 4, /* hi + lo bfloat16 split, 3 products (~16-bit)             */
   PSDP_FMT_BF16X9 = 5, /* hi + mid + lo bfloat16 split, 9 products (~fp32)         */
-  PSDP_FMT_E4M3 = 6    /* FP8 e4m3 operand values, per-tensor scale 2^8 (|a| <= 1  */
+  PSDP_FMT_E4M3 = 6,   /* FP8 e4m3 operand values, per-tensor scale 2^8 (|a| <= 1  */
                        /* after the bound), products on the binary16 MFMA          */
+  PSDP_FMT_F16MX = 7   /* hi binary16 of 2^8 v for hi.hi, e4m3 copies of hi and lo */
+                       /* with one power-of-two scale per 32 values (MX blocks)    */
+                       /* for the cross terms on the scaled FP8 MFMA: 2 binary16-  */
+                       /* MFMA equivalents per k instead of FP16x3's 3 (~15-bit)   */
 } psdp_format_t;
 
 /* Coefficient table of the composite polynomial chain. */
@@ -83,7 +87,8 @@ typedef enum psdp_method {
   PSDP_METHOD_COMPOSITE_FP16 = 1,          /* FP16 table, FP16 operands everywhere      */
   PSDP_METHOD_COMPOSITE_FP32_EMULATED = 2, /* FP32 table, BF16x9 everywhere             */
   PSDP_METHOD_MIXED = 3,                   /* FP16 table, FP16x3 on steps 0-3, FP16 rest */
-  PSDP_METHOD_CUSTOM = 4                   /* table + formats taken from psdp_config_t  */
+  PSDP_METHOD_CUSTOM = 4,                  /* table + formats taken from psdp_config_t  */
+  PSDP_METHOD_MIXED_MX = 5                 /* FP16 table, F16MX on steps 0-3, FP16 rest  */
 } psdp_method_t;
 
 #define PSDP_MAX_STEPS 16

@@ -150,6 +150,41 @@ def round_e4m3(x):
     return out
 
 
+MX_BLOCK = 32
+
+
+def mx_e4m3(x, axis):
+    """FP8 e4m3 values with one power-of-two scale per 32 consecutive entries
+    along ``axis`` (MX blocks, zero-padded at the end), returned dequantized
+    as float32.  A block whose largest magnitude has float32 exponent field E
+    gets the E8M0 scale byte b = max(E - 7, 0) (scale 2^(b - 127), so every
+    scaled value is below 256: no saturation), an all-zero block b = 127; each
+    value is round_e4m3(x / scale) * scale.  No reference counterpart: the
+    library's FMT_F16MX cross-term pieces (its own format)."""
+    x = np.asarray(x, dtype=np.float32)
+    xm = np.moveaxis(x, axis, -1)
+    n = xm.shape[-1]
+    pad = (-n) % MX_BLOCK
+    xp = np.pad(xm, [(0, 0)] * (xm.ndim - 1) + [(0, pad)])
+    blk = xp.reshape(xp.shape[:-1] + (-1, MX_BLOCK))
+    m = np.max(np.abs(blk), axis=-1, keepdims=True).astype(np.float32)
+    E = (m.view(np.uint32) >> 23).astype(np.int64)
+    b = np.where(m == 0, 127, np.maximum(E - 7, 0))
+    scaled = blk.astype(np.float64) * np.exp2((127 - b).astype(np.float64))
+    q = round_e4m3(scaled.astype(np.float32)).astype(np.float64) * np.exp2((b - 127).astype(np.float64))
+    out = q.astype(np.float32).reshape(xp.shape)[..., :n]
+    return np.moveaxis(out, -1, axis)
+
+
+def mx_split(x, axis):
+    """FMT_F16MX pieces of x: w = 2^8 x, hi = f16(w) (binary16 MFMA operand),
+    and the MX e4m3 copies of w and of lo = w - hi (blocks along ``axis``,
+    the k index of the product the operand enters)."""
+    w = np.asarray(x, dtype=np.float32) * np.float32(F16X3_SCALE)
+    hi = round_f16(w)
+    return [hi, mx_e4m3(w, axis), mx_e4m3(w - hi, axis)]
+
+
 def split_operand(x, fmt):
     """Return the list of float32 pieces a GEMM operand is stored as."""
     x = np.asarray(x, dtype=np.float32)
@@ -184,7 +219,10 @@ def emulated_product(x, y, fmt):
     (products of two 16-bit operands are exact in float32)."""
     if fmt == "f64":
         return np.asarray(x, np.float64) @ np.asarray(y, np.float64)
-    px, py = split_operand(x, fmt), split_operand(y, fmt)
+    if fmt == "f16mx":  # MX blocks along k: columns of x, rows of y
+        px, py = mx_split(x, 1), mx_split(y, 0)
+    else:
+        px, py = split_operand(x, fmt), split_operand(y, fmt)
     mm = lambda a, b: (a @ b).astype(np.float32)
     if fmt in ("f32", "f16", "bf16"):
         return mm(px[0], py[0])
@@ -192,6 +230,9 @@ def emulated_product(x, y, fmt):
         return mm(px[0], py[0]) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
     if fmt == "f16x3":
         acc = mm(px[1], py[0]) + mm(px[0], py[1])
+        return (acc + mm(px[0], py[0])) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
+    if fmt == "f16mx":
+        acc = mm(px[1], py[2]) + mm(px[2], py[1])  # cross terms from the e4m3 copies
         return (acc + mm(px[0], py[0])) * np.float32(1.0 / (F16X3_SCALE * F16X3_SCALE))
     if fmt == "bf16x3":
         return mm(px[0], py[0]) + (mm(px[0], py[1]) + mm(px[1], py[0]))
@@ -822,6 +863,11 @@ PINNING = {
     "eig_FP64_psd_staged": "equals eig_FP64_psd (scipy eigh, driver evd) and the closed form on random and "
                            "diagonal inputs; eigenvalues equal numpy eigvalsh",
     "GlibcRandom": "first outputs after srand(1) (1804289383, 846930886, ...) and libc rand()",
+    "mx_e4m3": "KATs (exact grid values, ties, zero and tiny blocks), brute force of every value against the "
+               "nearest e4m3 code times the frexp-derived block scale, axis/transpose and ragged-end padding; "
+               "the GPU's v_cvt_pk_fp8_f32 rounding matches round_e4m3 bit for bit",
+    "emulated_product(f16mx)": "error band between FP16x3 and binary16 with planted mistakes (no cross terms, "
+                               "hi8.hi8 pairing, wrong block axis) outside it; GPU products match to 1e-6",
     "round_e4m3": "OCP E4M3FN grid KATs (448, 2^-6, 2^-9, ties), brute force over all 256 codes, "
                   "torch float8_e4m3fn casts in range",
     "lanczos_eigenpairs": "5x5 KAT (bound >= 7.0890, lanczos_test.hpp:44-84); full-length runs on small "

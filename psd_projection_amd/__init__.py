@@ -35,26 +35,27 @@ LIB_PATH = _PKG / "libpsdp.so"
 # psdp_status_t
 OK, ERR_INVALID_ARG, ERR_NOMEM, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_DEVICE = range(6)
 # psdp_format_t
-FMT_F32, FMT_F16, FMT_F16X3, FMT_BF16, FMT_BF16X3, FMT_BF16X9, FMT_E4M3 = range(7)
+FMT_F32, FMT_F16, FMT_F16X3, FMT_BF16, FMT_BF16X3, FMT_BF16X9, FMT_E4M3, FMT_F16MX = range(8)
 FORMATS = {"f32": FMT_F32, "f16": FMT_F16, "f16x3": FMT_F16X3, "bf16": FMT_BF16,
-           "bf16x3": FMT_BF16X3, "bf16x9": FMT_BF16X9, "e4m3": FMT_E4M3}
+           "bf16x3": FMT_BF16X3, "bf16x9": FMT_BF16X9, "e4m3": FMT_E4M3, "f16mx": FMT_F16MX}
 # psdp_table_t
 TABLE_FP32, TABLE_FP16 = 0, 1
 # psdp_method_t
-METHOD_FP32, METHOD_FP16, METHOD_FP32_EMULATED, METHOD_MIXED, METHOD_CUSTOM = range(5)
+METHOD_FP32, METHOD_FP16, METHOD_FP32_EMULATED, METHOD_MIXED, METHOD_CUSTOM, METHOD_MIXED_MX = range(6)
 METHODS = {"composite_FP32": METHOD_FP32, "composite_FP16": METHOD_FP16,
            "composite_FP32_emulated": METHOD_FP32_EMULATED, "mixed": METHOD_MIXED,
-           "custom": METHOD_CUSTOM}
+           "custom": METHOD_CUSTOM, "mixed_mx": METHOD_MIXED_MX}
 # The MEX gateway's method strings (MATLAB/psd_projection_MATLAB.cu:112-166,
 # README.md:56) plus the shipped schedule; psdp_project_method dispatches them.
-MEX_METHODS = ("composite_FP16", "composite_FP32", "composite_FP32_emulated", "eig_FP64", "eig_FP32", "mixed")
+MEX_METHODS = ("composite_FP16", "composite_FP32", "composite_FP32_emulated", "eig_FP64", "eig_FP32", "mixed",
+               "mixed_mx")
 EVD_METHODS = ("eig_FP64", "eig_FP32")
 # psdp_bound_mode_t
 BOUND_REFERENCE, BOUND_TIGHT = 0, 1
 MAX_STEPS = 16
 # psdp_prof_kind_t
 PROF_KINDS = {0: "prod_f32", 1: "prod_f16", 2: "prod_f16x3", 3: "prod_bf16", 4: "prod_bf16x3",
-              5: "prod_bf16x9", 6: "prod_e4m3", 8: "lanczos", 9: "prep", 10: "eigenpairs", 11: "lobpcg",
+              5: "prod_bf16x9", 6: "prod_e4m3", 7: "prod_f16mx", 8: "lanczos", 9: "prep", 10: "eigenpairs", 11: "lobpcg",
               12: "rank_update", 13: "graph"}
 # psdp_ritz_residual_t
 RITZ_RESIDUAL_FIRST_ROW, RITZ_RESIDUAL_LAST_ROW = 0, 1

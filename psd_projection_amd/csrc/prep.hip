@@ -104,7 +104,8 @@ __global__ __launch_bounds__(256) void sym_pad_kernel<double>(const double* __re
 
 struct SplitArgs {
   float* a32;            // in: unscaled M; out: scaled iterate A0 (float32)
-  int64_t count;         // npad * npad
+  int64_t count;         // npad * ld (a multiple of 4096)
+  int64_t ld;            // row stride (FMT_F16MX blocks and scales)
   const double* inv;     // device scalar 1/s (null -> 1)
   int fmt_a;             // operand format of filter step 0
   uint16_t* ap[3];
@@ -113,14 +114,37 @@ struct SplitArgs {
   uint16_t* xp[3];
 };
 
+// Four consecutive values of one row (index i, a multiple of 4) in format
+// fmt; FMT_F16MX needs the eight lanes of each 32-value block together.
+__device__ __forceinline__ void store_operand4(int fmt, const float v[4], int64_t i, float* f32,
+                                               uint16_t* const* pieces);
+__device__ __forceinline__ void store_quad(int fmt, const float v[4], int64_t i, int64_t ld, float* f32,
+                                           uint16_t* const* pieces) {
+  if (fmt == FMT_F16MX) {
+    uint16_t hi[4];
+    uint32_t h8, l8, sc;
+    mx_split4(v, hi, h8, l8, sc);
+    uint16_t* const pc[3] = {pieces[0], pieces[1], pieces[2]};
+    mx_store4(pc, i / ld, (int)(i % ld), ld, hi, h8, l8, sc);
+    return;
+  }
+  store_operand4(fmt, v, i, f32, pieces);
+}
+
+// One quad per thread and step; a wave's 64 quads are in range together
+// (count is a multiple of 4096), so the FMT_F16MX lane groups stay whole.
 __global__ __launch_bounds__(256) void scale_split_kernel(SplitArgs s) {
   const double inv = s.inv ? *s.inv : 1.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.count; i += stride) {
-    const float v = (float)((double)s.a32[i] * inv);
-    s.a32[i] = v;
-    if (s.fmt_a != FMT_F32) store_operand(s.fmt_a, v, i, nullptr, s.ap);
-    store_operand(s.fmt_x, v, i, s.x32, s.xp);
+  const int64_t quads = s.count / 4;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < quads; t += stride) {
+    const int64_t i = 4 * t;
+    const f32x4 a = *(const __attribute__((address_space(1))) f32x4*)gbl(s.a32 + i);
+    const float v[4] = {(float)((double)a[0] * inv), (float)((double)a[1] * inv), (float)((double)a[2] * inv),
+                        (float)((double)a[3] * inv)};
+    *(__attribute__((address_space(1))) f32x4*)gbl(s.a32 + i) = f32x4{v[0], v[1], v[2], v[3]};
+    if (s.fmt_a != FMT_F32) store_quad(s.fmt_a, v, i, s.ld, nullptr, s.ap);
+    store_quad(s.fmt_x, v, i, s.ld, s.x32, s.xp);
   }
 }
 
@@ -188,8 +212,8 @@ __global__ __launch_bounds__(256) void scale_split_batched_kernel(const SplitJob
     const float v[4] = {(float)((double)a[0] * inv), (float)((double)a[1] * inv), (float)((double)a[2] * inv),
                         (float)((double)a[3] * inv)};
     *(__attribute__((address_space(1))) f32x4*)gbl(J.a32 + i) = f32x4{v[0], v[1], v[2], v[3]};
-    if (fmt_a != FMT_F32) store_operand4(fmt_a, v, i, nullptr, ap);
-    store_operand4(fmt_x, v, i, J.x32, xp);
+    if (fmt_a != FMT_F32) store_quad(fmt_a, v, i, J.ld, nullptr, ap);
+    store_quad(fmt_x, v, i, J.ld, J.x32, xp);
   }
 }
 
@@ -257,11 +281,13 @@ hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M
   return hipGetLastError();
 }
 
-hipError_t launch_scale_split(float* a32, int64_t count, const double* inv, int fmt_a, uint16_t* const ap[3],
-                              int fmt_x, float* x32, uint16_t* const xp[3], hipStream_t st) {
+hipError_t launch_scale_split(float* a32, int64_t count, int64_t ld, const double* inv, int fmt_a,
+                              uint16_t* const ap[3], int fmt_x, float* x32, uint16_t* const xp[3], hipStream_t st) {
+  if (count % 4096 != 0 || ld % 32 != 0) return hipErrorInvalidValue;
   SplitArgs s;
   s.a32 = a32;
   s.count = count;
+  s.ld = ld;
   s.inv = inv;
   s.fmt_a = fmt_a;
   s.fmt_x = fmt_x;
@@ -270,7 +296,7 @@ hipError_t launch_scale_split(float* a32, int64_t count, const double* inv, int 
     s.ap[i] = ap[i];
     s.xp[i] = xp[i];
   }
-  int64_t blocks = (count + 255) / 256;
+  int64_t blocks = (count / 4 + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(scale_split_kernel, dim3((unsigned)blocks), dim3(256), 0, st, s);
   return hipGetLastError();

@@ -36,7 +36,8 @@ hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc*
                                       int ntiles, int grid, hipStream_t s);
 hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
                           hipStream_t st);
-hipError_t launch_scale_split(float* a32, int64_t count, const double* inv, int fmt_a, uint16_t* const ap[3],
+hipError_t launch_scale_split(float* a32, int64_t count, int64_t ld, const double* inv, int fmt_a,
+                              uint16_t* const ap[3],
                               int fmt_x, float* x32, uint16_t* const xp[3], hipStream_t st);
 hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, int max_iter, double tol,
                           uint64_t seed, double margin, int tight, LanczosState* st, const LanczosVecs& v,
@@ -161,6 +162,15 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
       for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 4 ? FMT_F16X3 : FMT_F16;
       s.final_fmt = FMT_F16;
       break;
+    case PSDP_METHOD_MIXED_MX:
+      // The mixed schedule with the split steps' cross terms on the scaled
+      // FP8 MFMA (FMT_F16MX: two binary16-MFMA equivalents per k instead of
+      // three); measured rel-F error 6.0-6.3e-4 against 5.1-5.3e-4 for mixed
+      // (n = 1024-4096, three spectra, CPU emulation of both).
+      fill_table(PSDP_TABLE_FP16, s);
+      for (int i = 0; i < s.nsteps; ++i) s.fmt[i] = i < 4 ? FMT_F16MX : FMT_F16;
+      s.final_fmt = FMT_F16;
+      break;
     case PSDP_METHOD_CUSTOM:
       if (cfg->table != PSDP_TABLE_FP32 && cfg->table != PSDP_TABLE_FP16)
         return fail(PSDP_ERR_INVALID_ARG, "unknown coefficient table %d", cfg->table);
@@ -175,6 +185,13 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
         return fail(PSDP_ERR_INVALID_ARG, "unknown final format %d", cfg->final_format);
       s.final_fmt = cfg->final_format;
       s.final_avg = cfg->average_products && avg_capable(s.final_fmt);
+      // FMT_F16MX pieces are written by the 16-bit direct-to-LDS kernels only
+      for (int i = 0; i < s.nsteps; ++i) {
+        const int fn = i + 1 < s.nsteps ? s.fmt[i + 1] : s.final_fmt;
+        if (fn == FMT_F16MX && (s.fmt[i] == FMT_F32 || s.avg[i]))
+          return fail(PSDP_ERR_UNSUPPORTED, "step %d: a float32 or averaged step cannot be followed by F16MX", i);
+      }
+      if (s.final_fmt == FMT_F16MX && s.final_avg) return fail(PSDP_ERR_UNSUPPORTED, "F16MX products are not averaged");
       break;
     default:
       return fail(PSDP_ERR_INVALID_ARG, "unknown method %d", cfg->method);
@@ -183,7 +200,16 @@ static psdp_status_t build_schedule(const psdp_config_t* cfg, Schedule& s) {
 }
 
 static int pieces_of(int fmt) {
-  return fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : (fmt == FMT_F32 ? 0 : 1));
+  return fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 || fmt == FMT_F16MX ? 3 : (fmt == FMT_F32 ? 0 : 1));
+}
+
+// Bytes of piece q of an operand in format fmt, rows x ld 2-byte elements:
+// FMT_F16MX's piece 2 holds one uint16 scale pair per row and 32-column
+// block ((ld / 32) x ld of them).
+static size_t piece_bytes(int fmt, int q, int64_t rows, int64_t ld) {
+  if (q >= pieces_of(fmt)) return 0;
+  if (fmt == FMT_F16MX && q == 2) return ((size_t)(ld / MX_BLOCK) * ld * 2 + 255) & ~(size_t)255;
+  return ((size_t)rows * ld * 2 + 255) & ~(size_t)255;
 }
 
 // ---------------------------------------------------------------- handle
@@ -420,10 +446,11 @@ static constexpr size_t kNone = ~(size_t)0;
 // operand piece the format reads and every buffer the epilogue writes must be
 // present, so that a planning mistake becomes an error instead of a fault.
 static bool desc_ok(const ProdDesc& d, int fmt, const char** why) {
-  const int np = fmt == FMT_F32 ? 1 : (fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : 1));
+  const int np = fmt == FMT_F32 ? 1 : pieces_of(fmt);
   for (int i = 0; i < np; ++i)
     if (!d.x[i] || !d.y[i]) return *why = "missing operand piece", false;
   if (d.npad <= 0 || d.npad % 64 || d.ld < d.npad || d.n > d.npad) return *why = "bad sizes", false;
+  if ((fmt == FMT_F16MX || d.out_fmt == FMT_F16MX) && d.ld % MX_BLOCK) return *why = "ld not a multiple of 32", false;
   switch (d.mode) {
     case EPI_STORE:
       if (!d.c32 && d.out_fmt == FMT_F32) return *why = "no float32 destination", false;
@@ -439,7 +466,7 @@ static bool desc_ok(const ProdDesc& d, int fmt, const char** why) {
       return *why = "bad mode", false;
   }
   if (d.out_fmt != FMT_F32) {
-    const int no = d.out_fmt == FMT_F16X3 || d.out_fmt == FMT_BF16X3 ? 2 : (d.out_fmt == FMT_BF16X9 ? 3 : 1);
+    const int no = pieces_of(d.out_fmt);
     for (int i = 0; i < no; ++i)
       if (!d.outp[i]) return *why = "missing output piece", false;
   }
@@ -472,6 +499,19 @@ struct Plan {
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Piece sizes of the iterate A (every step format and the final one) and of
+// A^2 / A^3 (step formats), per piece index.
+static void piece_sizes(const Schedule& sch, int64_t rows, int64_t ld, size_t ab[3], size_t sb[3]) {
+  for (int q = 0; q < 3; ++q) {
+    ab[q] = piece_bytes(sch.final_fmt, q, rows, ld);
+    sb[q] = 0;
+    for (int i = 0; i < sch.nsteps; ++i) {
+      ab[q] = std::max(ab[q], piece_bytes(sch.fmt[i], q, rows, ld));
+      sb[q] = std::max(sb[q], piece_bytes(sch.fmt[i], q, rows, ld));
+    }
+  }
+}
 
 static int choose_sub(int64_t n) { return n > 1024 ? 4 : 2; }
 
@@ -520,16 +560,18 @@ static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
     pp = std::max(pp, pieces_of(p.sch.fmt[i]));
   }
   const int px = pieces_of(p.sch.final_fmt);
-  const size_t f32b = align256((size_t)p.N * 4), u16b = align256((size_t)p.N * 2);
+  const size_t f32b = align256((size_t)p.N * 4);
   size_t off = 0;
   p.off_a32 = off, off += f32b;
   p.off_p3 = off, off += f32b;
   if (any_f32) p.off_p2 = off, off += f32b;
   if (p.sch.final_fmt == FMT_F32) p.off_x32 = off, off += f32b;
-  for (int i = 0; i < pa; ++i) p.off_ap[i] = off, off += u16b;
-  for (int i = 0; i < pp; ++i) p.off_p2p[i] = off, off += u16b;
-  for (int i = 0; i < pp; ++i) p.off_p3p[i] = off, off += u16b;
-  for (int i = 0; i < px; ++i) p.off_xp[i] = off, off += u16b;
+  size_t ab[3], sb[3];
+  piece_sizes(p.sch, p.npad, p.ld, ab, sb);
+  for (int i = 0; i < pa; ++i) p.off_ap[i] = off, off += ab[i];
+  for (int i = 0; i < pp; ++i) p.off_p2p[i] = off, off += sb[i];
+  for (int i = 0; i < pp; ++i) p.off_p3p[i] = off, off += sb[i];
+  for (int i = 0; i < px; ++i) p.off_xp[i] = off, off += piece_bytes(p.sch.final_fmt, i, p.npad, p.ld);
   p.off_vec = off;
   off += align256((size_t)p.npad * 8) * 6 + align256((size_t)p.npad * 8 * p.lanczos_iters);
   p.total = off;
@@ -659,7 +701,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
   }
   {
     ProfScope ps(h, PSDP_PROF_PREP);
-    HIPCHK(launch_scale_split(b.a32, p.N, &st->inv_scale, sch.fmt[0], b.ap, sch.final_fmt, b.x32, b.xp, s));
+    HIPCHK(launch_scale_split(b.a32, p.N, p.ld, &st->inv_scale, sch.fmt[0], b.ap, sch.final_fmt, b.x32, b.xp, s));
   }
 
   for (int i = 0; i < sch.nsteps; ++i) {
@@ -986,7 +1028,7 @@ psdp_status_t psdp_autotune(psdp_handle_t h, int64_t n, const psdp_config_t* cfg
   std::sort(kinds.begin(), kinds.end());
   kinds.erase(std::unique(kinds.begin(), kinds.end()), kinds.end());
   for (auto [fmt, avg] : kinds) {
-    HIPCHK(launch_scale_split(b.a32, p.N, &h->st->inv_scale, fmt == FMT_F32 ? FMT_F32 : fmt, b.ap,
+    HIPCHK(launch_scale_split(b.a32, p.N, p.ld, &h->st->inv_scale, fmt == FMT_F32 ? FMT_F32 : fmt, b.ap,
                               p.sch.final_fmt, b.x32, b.xp, s));
     ProdDesc d = base_desc(p);
     set_operand(d.x, fmt, b.a32, b.ap);
@@ -1259,8 +1301,8 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
     HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, ld, npad, s));
     HIPCHK(launch_sym_pad(Y, 0, n, (int)n, my, ld, npad, s));
     uint16_t* none[3] = {nullptr, nullptr, nullptr};
-    HIPCHK(launch_scale_split(mx, N, nullptr, format, px, FMT_F32, mx, none, s));
-    HIPCHK(launch_scale_split(my, N, nullptr, format, py, FMT_F32, my, none, s));
+    HIPCHK(launch_scale_split(mx, N, ld, nullptr, format, px, FMT_F32, mx, none, s));
+    HIPCHK(launch_scale_split(my, N, ld, nullptr, format, py, FMT_F32, my, none, s));
   }
   ProdDesc d;
   memset(&d, 0, sizeof d);
@@ -1365,15 +1407,17 @@ static size_t layout_block(const Schedule& sch, int n, int iters, int big_n, siz
   q.npad = (n + bt - 1) / bt * bt;
   q.iters = std::min(iters, n);
   const int64_t N = (int64_t)q.npad * q.npad;
-  const size_t f32b = align256((size_t)N * 4), u16b = align256((size_t)N * 2);
+  const size_t f32b = align256((size_t)N * 4);
   q.a32 = off, off += f32b;
   q.p3 = off, off += f32b;
   if (any_f32) q.p2 = off, off += f32b;
   if (sch.final_fmt == FMT_F32) q.x32 = off, off += f32b;
-  for (int i = 0; i < pa; ++i) q.ap[i] = off, off += u16b;
-  for (int i = 0; i < pp; ++i) q.p2p[i] = off, off += u16b;
-  for (int i = 0; i < pp; ++i) q.p3p[i] = off, off += u16b;
-  for (int i = 0; i < px; ++i) q.xp[i] = off, off += u16b;
+  size_t ab[3], sb[3];
+  piece_sizes(sch, q.npad, q.npad, ab, sb);
+  for (int i = 0; i < pa; ++i) q.ap[i] = off, off += ab[i];
+  for (int i = 0; i < pp; ++i) q.p2p[i] = off, off += sb[i];
+  for (int i = 0; i < pp; ++i) q.p3p[i] = off, off += sb[i];
+  for (int i = 0; i < px; ++i) q.xp[i] = off, off += piece_bytes(sch.final_fmt, i, q.npad, q.npad);
   q.vec = off;
   off += align256((size_t)q.npad * 8) * 6 + align256((size_t)q.npad * 8 * q.iters);
   return off;
@@ -1417,6 +1461,7 @@ static void block_descs(const Schedule& sch, const BlockBufs& q, char* base, con
   pj = PadJob{X, q.n, q.npad, B.a32};
   S.a32 = B.a32;
   S.count = (int64_t)q.npad * q.npad;
+  S.ld = q.npad;
   S.inv = &st->inv_scale;
   S.x32 = B.x32;
   for (int i = 0; i < 3; ++i) {
@@ -2341,7 +2386,7 @@ psdp_status_t psdp_free(void* p) {
 
 // MATLAB/psd_projection_MATLAB.cu:112-166: the five method strings of the
 // MEX gateway (README.md:56), each an in-place float64 projection that
-// returns when done; "mixed" adds the shipped schedule.
+// returns when done; "mixed" and "mixed_mx" add the library's schedules.
 psdp_status_t psdp_project_method(psdp_handle_t h, const char* method, double* mat, int64_t n,
                                   double* eigenvalues) {
   if (!method) return fail(PSDP_ERR_INVALID_ARG, "null method name");
@@ -2352,9 +2397,10 @@ psdp_status_t psdp_project_method(psdp_handle_t h, const char* method, double* m
   if (m == "eig_FP64") return psdp_eig_fp64_psd(h, mat, n, eigenvalues);
   if (m == "eig_FP32") return psdp_eig_fp32_psd(h, mat, n, eigenvalues);
   if (m == "mixed") return dropin(h, mat, n, nullptr, 0, PSDP_METHOD_MIXED, 1);
+  if (m == "mixed_mx") return dropin(h, mat, n, nullptr, 0, PSDP_METHOD_MIXED_MX, 1);
   return fail(PSDP_ERR_INVALID_ARG,
               "unknown method '%s' (composite_FP16, composite_FP32, composite_FP32_emulated, eig_FP64, eig_FP32, "
-              "mixed)",
+              "mixed, mixed_mx)",
               method);
 }
 

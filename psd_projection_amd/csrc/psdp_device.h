@@ -13,6 +13,17 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
+// Global-memory view of a pointer that was read from device memory (a field
+// of a launch descriptor).  Without it the compiler cannot prove the address
+// space and emits FLAT accesses, which also count on the LDS counter: every
+// wait for an LDS read would then wait for all outstanding stores as well.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// ---------------------------------------------------------------------------
 // Scalar conversions (round to nearest even).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
@@ -80,6 +91,76 @@ __device__ __forceinline__ void split_value(int fmt, float v, uint16_t out[3]) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// FMT_F16MX pieces (layout in psdp_internal.h).  Eight consecutive lanes
+// hold the four-value quads of one 32-value block; every lane of the eight
+// must call these together (the block maxima go through lane shuffles).
+// ---------------------------------------------------------------------------
+// E8M0 scale of a block whose largest magnitude has float bits m (magnitudes
+// compare as unsigned integers): 2^(E - 7) for exponent field E, so the
+// block's values scale below 256; an all-zero block gets 2^0.
+__device__ __forceinline__ uint32_t mx_scale_byte(uint32_t m) {
+  if (m == 0) return 127u;
+  const int b = (int)(m >> 23) - 7;
+  return (uint32_t)(b < 0 ? 0 : b);
+}
+// 2^(127 - b): divides by the scale exactly (b <= 248 for finite blocks).
+__device__ __forceinline__ float mx_inv_scale(uint32_t b) { return __uint_as_float((254u - b) << 23); }
+
+__device__ __forceinline__ uint32_t max_of_8_lanes(uint32_t v) {
+  v = max(v, (uint32_t)__shfl_xor((int)v, 1, 64));
+  v = max(v, (uint32_t)__shfl_xor((int)v, 2, 64));
+  v = max(v, (uint32_t)__shfl_xor((int)v, 4, 64));
+  return v;
+}
+
+// Four consecutive values v (of one row, column a multiple of 4) to their
+// binary16 hi pieces, their packed e4m3 bytes of hi and lo (h8, l8: value e
+// in byte e) and the block's scale pair (hi's | lo's << 8).
+__device__ __forceinline__ void mx_split4(const float v[4], uint16_t hi[4], uint32_t& h8, uint32_t& l8,
+                                          uint32_t& sc) {
+  float w[4], lo[4];
+  uint32_t mw = 0, ml = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    w[e] = v[e] * F16X3_SCALE;
+    hi[e] = f32_to_f16_bits(w[e]);
+    lo[e] = w[e] - f16_bits_to_f32(hi[e]);
+    mw = max(mw, __float_as_uint(w[e]) & 0x7fffffffu);
+    ml = max(ml, __float_as_uint(lo[e]) & 0x7fffffffu);
+  }
+  const uint32_t bw = mx_scale_byte(max_of_8_lanes(mw)), bl = mx_scale_byte(max_of_8_lanes(ml));
+  const float iw = mx_inv_scale(bw), il = mx_inv_scale(bl);
+  int ph = 0, pl = 0;
+  ph = __builtin_amdgcn_cvt_pk_fp8_f32(w[0] * iw, w[1] * iw, ph, false);
+  ph = __builtin_amdgcn_cvt_pk_fp8_f32(w[2] * iw, w[3] * iw, ph, true);
+  pl = __builtin_amdgcn_cvt_pk_fp8_f32(lo[0] * il, lo[1] * il, pl, false);
+  pl = __builtin_amdgcn_cvt_pk_fp8_f32(lo[2] * il, lo[3] * il, pl, true);
+  h8 = (uint32_t)ph;
+  l8 = (uint32_t)pl;
+  sc = bw | (bl << 8);
+}
+
+// Byte offset of element (r, c) in piece 1 of FMT_F16MX (hi's byte; lo's is
+// 32 bytes further), and the index of its block's scale in piece 2.
+__device__ __forceinline__ int64_t mx_byte(int64_t r, int c, int64_t ld) {
+  return 2 * r * ld + 64 * (c >> 5) + (c & 31);
+}
+__device__ __forceinline__ int64_t mx_scale_index(int64_t r, int c, int64_t ld) { return (int64_t)(c >> 5) * ld + r; }
+
+// Stores the quad of mx_split4 at (r, c .. c + 3); the lane holding the
+// block's first quad also stores the scale pair.
+__device__ __forceinline__ void mx_store4(uint16_t* const pieces[3], int64_t r, int c, int64_t ld,
+                                          const uint16_t hi[4], uint32_t h8, uint32_t l8, uint32_t sc) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 pk = {(uint32_t)hi[0] | ((uint32_t)hi[1] << 16), (uint32_t)hi[2] | ((uint32_t)hi[3] << 16)};
+  *(__attribute__((address_space(1))) u32x2*)gbl(pieces[0] + r * ld + c) = pk;
+  char* b8 = (char*)pieces[1] + mx_byte(r, c, ld);
+  *(__attribute__((address_space(1))) uint32_t*)gbl((uint32_t*)b8) = h8;
+  *(__attribute__((address_space(1))) uint32_t*)gbl((uint32_t*)(b8 + 32)) = l8;
+  if ((c & 31) == 0) *gbl(pieces[2] + mx_scale_index(r, c, ld)) = (uint16_t)sc;
+}
+
 __device__ __forceinline__ int fmt_pieces(int fmt) {
   return fmt == FMT_F16X3 || fmt == FMT_BF16X3 ? 2 : (fmt == FMT_BF16X9 ? 3 : 1);
 }
@@ -100,16 +181,6 @@ __device__ __forceinline__ void store_operand(int fmt, float v, int64_t idx, flo
   if (np > 2) pieces[2][idx] = p[2];
 }
 
-// ---------------------------------------------------------------------------
-// Global-memory view of a pointer that was read from device memory (a field
-// of a launch descriptor).  Without it the compiler cannot prove the address
-// space and emits FLAT accesses, which also count on the LDS counter: every
-// wait for an LDS read would then wait for all outstanding stores as well.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
-  return (__attribute__((address_space(1))) T*)p;
-}
 
 // ---------------------------------------------------------------------------
 // Seeded counter-based uniform stream: element i of stream `seed` is

@@ -13,8 +13,24 @@ enum : int {
   FMT_BF16X3 = 4,  // hi + lo bfloat16 split, 3 products
   FMT_BF16X9 = 5,  // hi + mid + lo bfloat16 split, 9 products
   FMT_E4M3 = 6,    // 2^8 v rounded to the FP8 e4m3 grid, held in binary16 (binary16 MFMA)
-  FMT_COUNT = 7
+  FMT_F16MX = 7,   // hi binary16 of 2^8 v + MX-scaled e4m3 copies of hi and lo for the cross terms
+  FMT_COUNT = 8
 };
+
+// FMT_F16MX: w = 2^8 v, hi = f16(w), lo = w - hi (exact in float32).  The
+// product is hi.hi on the binary16 MFMA plus the cross terms hi.lo + lo.hi
+// from e4m3 copies on the scaled FP8 MFMA (v_mfma_scale_f32_32x32x64_f8f6f4),
+// which runs at twice the binary16 rate: two binary16-MFMA equivalents per
+// k instead of three.  Pieces (row stride ld, in 2-byte units):
+//   [0] hi as binary16 at row * ld + col;
+//   [1] e4m3 bytes: row r, 32-column block kb holds 64 bytes at byte
+//       2 * r * ld + 64 * kb: the block's 32 hi values, then its 32 lo values;
+//   [2] block scales (E8M0 exponents, value 2^(b - 127)), one uint16 per
+//       (row, block) at kb * ld + r: low byte hi's, high byte lo's.
+// A block's scale is 2^(E - 7) with E the exponent of its largest magnitude,
+// so every scaled value is below 256 (no saturation); an all-zero block has
+// scale 1.  Needs ld a multiple of 32.
+constexpr int MX_BLOCK = 32;
 
 // FMT_F16X3 splits s*v (s = 2^8, exact) into hi + lo binary16 pieces so that
 // lo stays out of the binary16 subnormal range for the entry sizes of a
@@ -123,6 +139,7 @@ struct PadJob {
 struct SplitJob {
   float* a32;
   int64_t count;
+  int64_t ld;  // row stride (FMT_F16MX blocks and scales)
   const double* inv;
   uint16_t* ap[3];
   float* x32;

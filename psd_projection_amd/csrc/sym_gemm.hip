@@ -64,6 +64,13 @@ struct FmtTraits<FMT_BF16X9> {
   using E = uint16_t;
   static constexpr int P = 3, BK = 32;
 };
+// Two 2-byte-per-k pieces per panel (hi binary16, the e4m3 hi/lo bytes) staged
+// like FP16x3's, plus the block scales (piece 2) of every slab.
+template <>
+struct FmtTraits<FMT_F16MX> {
+  using E = uint16_t;
+  static constexpr int P = 2, BK = 32;
+};
 
 template <int FMT, int SUB, bool AVG = false>
 struct Tiling {
@@ -245,7 +252,143 @@ __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, fl
   }
 }
 
-template <int BT, int NT = 256, int GMAX = 8>
+// FMT_F16MX outputs: the same two passes, with each 32-value block of a
+// stored row split together (mx_split4 over the eight lanes of its quads).
+// Pass 1 first leaves the final values of the staged rows in C.  In a
+// diagonal tile a block that contains the diagonal is stored partly by each
+// pass (row i at j >= i from row i of C, at j < i from column i), so the
+// staged rows' diagonal 32 x 32 blocks are made symmetric in C before any of
+// them is stored: both passes then split the same 32 values and write the
+// same scale.  Staged rows start at multiples of 32, so such a block is
+// always staged whole.
+__device__ __forceinline__ void mx_store1(uint16_t* const pieces[3], int64_t r, int c, int64_t ld, uint16_t hi,
+                                          uint32_t h8, uint32_t l8, int e) {
+  *gbl(pieces[0] + r * ld + c) = hi;
+  char* b8 = (char*)pieces[1] + mx_byte(r, c, ld);
+  *gbl(b8) = (char)(h8 >> (8 * e));
+  *gbl(b8 + 32) = (char)(l8 >> (8 * e));
+}
+
+template <int BT, int NT, int GMAX = 8>
+__device__ __forceinline__ void epilogue_store_rows_mx(float* C, int r0, int NR, float cscale, int row0, int col0,
+                                                       bool diag, int64_t ld,
+                                                       __attribute__((address_space(1))) float* c32,
+                                                       __attribute__((address_space(1))) const float* p3,
+                                                       uint16_t* const outp[3], bool poly, bool half_id, float ca,
+                                                       float cb, float cc) {
+  constexpr int CS = BT + 1;
+  constexpr int Q4 = BT / 4;
+  static_assert(Q4 % 8 == 0, "a 32-value block is 8 quads of one row");
+  const int tid = threadIdx.x;
+  const int nq = NR / 4;
+  constexpr int G0 = (NR_MAX<BT> * Q4 / NT) < 8 ? (NR_MAX<BT> * Q4 / NT > 0 ? NR_MAX<BT> * Q4 / NT : 1) : 8;
+  constexpr int G = G0 < GMAX ? G0 : GMAX;
+  // pass 1: final values (and, off the diagonal, their stores)
+#pragma unroll 1
+  for (int q0 = tid; q0 < NR * Q4; q0 += NT * G) {
+    f32x4 a4g[G], p4g[G];
+    if (poly) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int q = q0 + g * NT;
+        if (q < NR * Q4) {
+          const int64_t idx = (int64_t)(row0 + q / Q4) * ld + col0 + (q % Q4) * 4;
+          a4g[g] = *(const __attribute__((address_space(1))) f32x4*)(c32 + idx);
+          p4g[g] = *(const __attribute__((address_space(1))) f32x4*)(p3 + idx);
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g * NT;
+      if (q >= NR * Q4) break;  // uniform over each wave (NR * Q4 is a multiple of 64)
+      const int r = q / Q4, c0 = (q % Q4) * 4;
+      const int i = row0 + r, j0 = col0 + c0;
+      float w[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
+      if (poly) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float x = ca * a4g[g][e];
+          x = fmaf(cb, p4g[g][e], x);
+          x = fmaf(cc, w[e], x);
+          if (half_id) x = (x + (i == j0 + e ? 1.0f : 0.0f)) * 0.5f;
+          w[e] = x;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) C[r * CS + c0 + e] = w[e];
+      if (!diag) {
+        const int64_t idx = (int64_t)i * ld + j0;
+        if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
+        uint16_t hi[4];
+        uint32_t h8, l8, sc;
+        mx_split4(w, hi, h8, l8, sc);
+        mx_store4(outp, i, j0, ld, hi, h8, l8, sc);
+      }
+    }
+  }
+  __syncthreads();
+  if (diag) {
+    // lower half of the staged rows' diagonal 32 x 32 blocks := upper half
+    for (int e = tid; e < NR * 32; e += NT) {
+      const int r = e >> 5, R = r0 + r, c = (R & ~31) + (e & 31);
+      if (c < R) C[r * CS + c] = C[(c - r0) * CS + R];
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int q = tid; q < NR * Q4; q += NT) {
+      const int r = q / Q4, c0 = (q % Q4) * 4;
+      const int i = row0 + r, j0 = col0 + c0;
+      float w[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e];
+      uint16_t hi[4];
+      uint32_t h8, l8, sc;
+      mx_split4(w, hi, h8, l8, sc);
+      const int64_t idx = (int64_t)i * ld + j0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (i <= j0 + e) {
+          if (c32) c32[idx + e] = w[e];
+          mx_store1(outp, i, j0 + e, ld, hi[e], h8, l8, e);
+        }
+      }
+      if ((c0 & 31) == 0 && (j0 | 31) >= i) *gbl(outp[2] + mx_scale_index(i, j0, ld)) = (uint16_t)sc;
+    }
+  }
+  // pass 2: mirror block, columns of the tile (stored row j, columns i0 ..)
+#pragma unroll 2
+  for (int q = tid; q < BT * nq; q += NT) {
+    const int c = q / nq, rr = (q % nq) * 4;
+    const int j = col0 + c, i0 = row0 + rr;
+    float w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = C[(rr + e) * CS + c];
+    uint16_t hi[4];
+    uint32_t h8, l8, sc;
+    mx_split4(w, hi, h8, l8, sc);
+    const int64_t idx = (int64_t)j * ld + i0;
+    if (!diag) {
+      if (c32) *(__attribute__((address_space(1))) f32x4*)(c32 + idx) = f32x4{w[0], w[1], w[2], w[3]};
+      mx_store4(outp, j, i0, ld, hi, h8, l8, sc);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (i0 + e < j) {
+          if (c32) c32[idx + e] = w[e];
+          mx_store1(outp, j, i0 + e, ld, hi[e], h8, l8, e);
+        }
+      }
+      if ((rr & 31) == 0 && (i0 & ~31) < j) *gbl(outp[2] + mx_scale_index(j, i0, ld)) = (uint16_t)sc;
+    }
+  }
+}
+
+// MXOUT = false leaves out the FMT_F16MX store path (kernels that never write
+// it: the float32 and averaged products, whose registers it would crowd).
+template <int BT, int NT = 256, int GMAX = 8, bool MXOUT = true>
 __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
                                               float cscale = 1.0f) {
   constexpr int CS = BT + 1;
@@ -313,6 +456,13 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
 #define PSDP_EPI(OF)                                                                                      \
   epilogue_store_rows<BT, NT, OF, GMAX>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id, ca, \
                                   cb, cc)
+  if constexpr (MXOUT) {
+    if (out_fmt == FMT_F16MX) {
+      epilogue_store_rows_mx<BT, NT, GMAX>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id,
+                                           ca, cb, cc);
+      return;
+    }
+  }
   switch (out_fmt) {
     case FMT_F16: PSDP_EPI(FMT_F16); break;
     case FMT_F16X3: PSDP_EPI(FMT_F16X3); break;
@@ -531,7 +681,7 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
           }
     }
     __syncthreads();
-    epilogue_rows<TL::BT>(d, bi, bj, Ct, ph * TL::HR, TL::HR);
+    epilogue_rows<TL::BT, 256, 8, false>(d, bi, bj, Ct, ph * TL::HR, TL::HR);
     __syncthreads();
   }
 }
@@ -563,17 +713,22 @@ struct DmaTiling {
   // many L2 requests as 32-element slabs) where two such stages fit in LDS;
   // with producer waves two-piece operands also take 32-element slabs (the
   // fragments of 64 would not fit 256 registers per wave).
-  static constexpr bool FIT64 = 2 * (2 * P * BT * 128) <= LDS_BUDGET;
+  // FMT_F16MX stages the block scales of both panels after each stage's
+  // operand bytes: 2 bytes per row and 32-k block.
+  static constexpr bool MX = FMT == FMT_F16MX;
+  static constexpr bool FIT64 = 2 * (2 * P * BT * 128 + (MX ? 8 * BT : 0)) <= LDS_BUDGET;
   static constexpr int BK = (FIT64 && !(WS && P == 2)) ? 64 : 32;  // slab width (elements)
   static constexpr int RB = BK * 2;                        // row bytes
   static constexpr int TILE_BYTES = BT * RB;               // one (panel, piece)
   static constexpr int STAGE_BYTES = 2 * P * TILE_BYTES;   // X rows of bi, Y rows of bj
-  static constexpr int SFIT = LDS_BUDGET / STAGE_BYTES;
+  static constexpr int SCALE_BYTES = MX ? 2 * 2 * BT * (BK / 32) : 0;
+  static constexpr int STAGE_STRIDE = STAGE_BYTES + SCALE_BYTES;
+  static constexpr int SFIT = LDS_BUDGET / STAGE_STRIDE;
   static constexpr int S = SFIT >= 4 ? 4 : SFIT;           // pipeline depth (>= 2)
   static constexpr int L = STAGE_BYTES / 4096;             // loads per thread per stage
   static constexpr int BLOCKS = TILE_BYTES / 1024;         // 1 KB blocks per tile
   static constexpr int HR = BT < 128 ? BT : 128;
-  static constexpr int LDS_MAIN = S * STAGE_BYTES;
+  static constexpr int LDS_MAIN = S * STAGE_STRIDE;
   static constexpr int LDS_CTILE = HR * (BT + 1) * 4;
   static constexpr int LDS_BYTES = LDS_MAIN > LDS_CTILE ? LDS_MAIN : LDS_CTILE;
   static_assert(STAGE_BYTES % 4096 == 0, "stage must be a whole number of 4 KB loads");
@@ -1010,6 +1165,433 @@ __global__ __launch_bounds__(512) void sym_prod_dma8_kernel(ProdDesc d, const Ti
   sym_tile_dma8<FMT>(d, t.bi, t.bj, smem);
 }
 
+// ------------------------------------------------------------------ FMT_F16MX
+// Products of FMT_F16MX operands.  Per 32-k block and 32 x 32 accumulator:
+// one scaled FP8 MFMA whose 64 k are the block's hi.lo (first 32) and lo.hi
+// (last 32), then the block's two binary16 hi.hi MFMAs, all into one sum
+// (smallest terms first, as FP16x3's lo.hi, hi.lo, hi.hi).  The operand
+// pieces are staged like FP16x3's two pieces (same bytes per row and slab,
+// same swizzles and fragment reads), plus each slab's block scales.
+//
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e4m3 operands (measured on gfx950):
+// lane l holds row l % 32; bytes 0-15 of lanes l and l + 32 are the 32 k of
+// the first block, bytes 16-31 those of the second, A and B lanes pairing
+// byte for byte; the scale operand (low byte) of lane r + 32 b scales block
+// b of row r.  So A lane (r, h) passes hi bytes 16h.. then lo bytes 16h..
+// with its row's hi (h = 0) or lo (h = 1) scale, and B lane (c, h) passes lo
+// bytes then hi bytes with its row's lo (h = 0) or hi (h = 1) scale.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x16 mfma_mx(i32x8 a, i32x8 b, int sa, int sb, f32x16 c) {
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, sa, 0, sb);
+}
+// The 32 bytes of an e4m3 fragment from two 16-byte LDS reads, built where
+// they are read so that they land in one register tuple.
+__device__ __forceinline__ i32x8 cat16(u32x4 x, u32x4 y) {
+  return __builtin_bit_cast(i32x8, __builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// Scale words of a stage: thread t (mod the stage's word count) copies 4
+// bytes (two rows) of segment (panel g, block q): rows of the X panel of bi
+// for g = 0, of the Y panel of bj for g = 1.  With more threads than words
+// the extra threads copy the same words again (same bytes to the same place),
+// so that every wave issues the same number of loads.
+template <int BT, int NQ, int NT>
+struct MxScaleLoad {
+  static constexpr int SCB = 2 * 2 * BT * NQ, SW = SCB / 4;
+  static_assert(SW % 64 == 0, "scale words per stage must fill whole waves");
+  const uint16_t* src;
+  int dst;  // wave-uniform byte offset in the stage's scale area
+  __device__ __forceinline__ MxScaleLoad(const ProdDesc& d, int row0, int col0, int tid, int lane) {
+    const int sw = tid % SW;
+    const int seg = (4 * sw) / (2 * BT), row = ((4 * sw) % (2 * BT)) / 2;
+    const int g = seg / NQ, q = seg % NQ;
+    src = (const uint16_t*)(g == 0 ? d.x[2] : d.y[2]) + (int64_t)q * d.ld + (g == 0 ? row0 : col0) + row;
+    dst = __builtin_amdgcn_readfirstlane(4 * (sw - lane));
+  }
+  __device__ __forceinline__ void issue(int kt, char* area, int64_t ld) const {
+    __builtin_amdgcn_global_load_lds((const void*)(src + (int64_t)kt * NQ * ld), (lds_ptr_t)(area + dst), 4, 0, 0);
+  }
+};
+
+// 4 waves, 2 x 2 of (16 SUB) x (16 SUB), M x M accumulators each (SUB = 2, 4).
+template <int SUB>
+__device__ __forceinline__ void sym_tile_dma_mx(const ProdDesc& d, int bi, int bj, char* smem) {
+  using DT = DmaTiling<FMT_F16MX, SUB, false>;
+  constexpr int P = 2, BK = DT::BK, RB = DT::RB, S = DT::S, L = DT::L, BT = DT::BT;
+  constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES, SST = DT::STAGE_STRIDE;
+  constexpr int NQ = BK / 32, LT = L + 1;  // blocks per slab, loads per thread per stage
+  static_assert((S - 1) * LT <= 63, "vmcnt range");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = (wave & 3) >> 1, wc = wave & 1;
+  const int64_t ld = d.ld;
+  const int row0 = bi * BT, col0 = bj * BT;
+
+  constexpr int CPR = RB / 16, RPB = 1024 / RB;
+  const uint16_t* gsrc[L];
+  int lofs[L], loff[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int q = wave * L + i;
+    const int gp = q / DT::BLOCKS, blk = q % DT::BLOCKS;
+    const int g = gp / P, p = gp % P;
+    const void* bx = p == 0 ? d.x[0] : d.x[1];
+    const void* by = p == 0 ? d.y[0] : d.y[1];
+    const void* base = g == 0 ? bx : by;
+    gsrc[i] = (const uint16_t*)base + (int64_t)((g == 0 ? row0 : col0) + blk * RPB) * ld;
+    lofs[i] = gp * TB + blk * 1024;
+    const int r = blk * RPB + lane / CPR;
+    loff[i] = (lane / CPR) * (int)ld + ((lane % CPR) ^ dma_swz<RB>(r)) * 8;
+  }
+  const MxScaleLoad<BT, NQ, 256> sload(d, row0, col0, tid, lane);
+#define PSDP_MX_ISSUE(KT, BUF)                                                                       \
+  {                                                                                                 \
+    _Pragma("unroll") for (int i = 0; i < L; ++i)                                                   \
+        __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (KT) * BK + loff[i]),              \
+                                         (lds_ptr_t)(smem + (BUF) * SST + lofs[i]), 16, 0, 0);      \
+    sload.issue(KT, smem + (BUF) * SST + SB, ld);                                                   \
+  }
+
+  constexpr int M = SUB / 2, KS = BK / 16;
+  f32x16 acc[M][M];
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
+
+  const int arow = (wr * DT::WT + (lane & 31)) * RB;
+  const int brow = (wc * DT::WT + (lane & 31)) * RB;
+  const int swzl = dma_swz<RB>(lane & 31);
+  const int h = lane >> 5;
+  const int KT = d.npad / BK;
+
+  // binary16 hi fragments per k-step, e4m3 fragments per block
+  u32x4 fa[KS][M], fb[KS][M], na[KS][M];
+  i32x8 fa8[NQ][M], fb8[NQ][M], na8[NQ][M];
+  int sa[NQ][M], sb[NQ][M], nsa[NQ][M];
+  auto frag_a = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + p * TB + arow + m * 32 * RB + 16 * ((ks * 2 + h) ^ swzl));
+  };
+  auto frag_b = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + (P + p) * TB + brow + m * 32 * RB + 16 * ((ks * 2 + h) ^ swzl));
+  };
+  // A: hi bytes (chunk of k-step 2q) then lo bytes (k-step 2q + 1); B: lo then hi
+  auto frag_a8 = [&](const char* stg, int q, int m) -> i32x8 {
+    return cat16(frag_a(stg, 2 * q, 1, m), frag_a(stg, 2 * q + 1, 1, m));
+  };
+  auto frag_b8 = [&](const char* stg, int q, int m) -> i32x8 {
+    return cat16(frag_b(stg, 2 * q + 1, 1, m), frag_b(stg, 2 * q, 1, m));
+  };
+  // scale bytes: A lane (r, h) takes byte h of its row's pair, B lane byte 1 - h
+  auto scl_a = [&](const char* stg, int q, int m) -> int {
+    return *(const unsigned char*)(stg + SB + (q * BT + wr * DT::WT + m * 32 + (lane & 31)) * 2 + h);
+  };
+  auto scl_b = [&](const char* stg, int q, int m) -> int {
+    return *(const unsigned char*)(stg + SB + ((NQ + q) * BT + wc * DT::WT + m * 32 + (lane & 31)) * 2 + 1 - h);
+  };
+
+#pragma unroll
+  for (int st = 0; st < S; ++st) PSDP_MX_ISSUE((st < KT ? st : KT - 1), st)
+  wait_vmcnt<(S - 1) * LT>();
+  lds_barrier();
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      fa[ks][m] = frag_a(smem, ks, 0, m);
+      fb[ks][m] = frag_b(smem, ks, 0, m);
+    }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      fa8[q][m] = frag_a8(smem, q, m);
+      fb8[q][m] = frag_b8(smem, q, m);
+      sa[q][m] = scl_a(smem, q, m);
+      sb[q][m] = scl_b(smem, q, m);
+    }
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+
+#define PSDP_MX_COLUMN(q, mj)                                                                     \
+  _Pragma("unroll") for (int mi = 0; mi < M; ++mi) {                                              \
+    f32x16 c = mfma_mx(fa8[q][mi], fb8[q][mj], sa[q][mi], sb[q][mj], acc[mi][mj]);                \
+    c = mfma32(fa[2 * (q)][mi], fb[2 * (q)][mj], c, FMT_F16);                                     \
+    acc[mi][mj] = mfma32(fa[2 * (q) + 1][mi], fb[2 * (q) + 1][mj], c, FMT_F16);                   \
+  }
+
+  constexpr int NREAD = 2 * KS * P * M;
+  constexpr int NMFMA = NQ * M * M * 3;
+  constexpr int MR = NMFMA / NREAD > 0 ? NMFMA / NREAD : 1;
+  for (int kt = 0; kt + 1 < KT; ++kt) {
+    wait_vmcnt<(S - 2) * LT>();          // own DMA of slab kt + 1 landed
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // own reads of buffer kt % S complete
+    lds_barrier();
+    PSDP_MX_ISSUE((kt + S < KT ? kt + S : KT - 1), kt % S)
+    const char* nxt = smem + ((kt + 1) % S) * SST;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int m = 0; m < M; ++m) na[ks][m] = frag_a(nxt, ks, 0, m);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        na8[q][m] = frag_a8(nxt, q, m);
+        nsa[q][m] = scl_a(nxt, q, m);
+      }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int mj = 0; mj < M; ++mj) {
+        PSDP_MX_COLUMN(q, mj)
+        fb[2 * q][mj] = frag_b(nxt, 2 * q, 0, mj);
+        fb[2 * q + 1][mj] = frag_b(nxt, 2 * q + 1, 0, mj);
+        fb8[q][mj] = frag_b8(nxt, q, mj);
+        sb[q][mj] = scl_b(nxt, q, mj);
+      }
+#pragma unroll
+    for (int g = 0; g < NREAD; ++g) {
+      if (g < L) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);            // DS read
+      __builtin_amdgcn_sched_group_barrier(0x008, MR, 0);            // MFMA
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int m = 0; m < M; ++m) fa[ks][m] = na[ks][m];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        fa8[q][m] = na8[q][m];
+        sa[q][m] = nsa[q][m];
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int mj = 0; mj < M; ++mj) { PSDP_MX_COLUMN(q, mj) }
+#undef PSDP_MX_COLUMN
+#undef PSDP_MX_ISSUE
+  __builtin_amdgcn_sched_barrier(0);
+  wait_vmcnt<0>();
+  lds_barrier();
+
+  float* Ct = (float*)smem;
+  constexpr int CS = BT + 1;
+  constexpr int PHASES = BT / DT::HR;
+#pragma unroll
+  for (int ph = 0; ph < PHASES; ++ph) {
+    if (PHASES == 1 || wr == ph) {
+#pragma unroll
+      for (int mi = 0; mi < M; ++mi)
+#pragma unroll
+        for (int mj = 0; mj < M; ++mj)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const int r = wr * DT::WT + mi * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3) - ph * DT::HR;
+            const int c = wc * DT::WT + mj * 32 + (lane & 31);
+            Ct[r * CS + c] = acc[mi][mj][e];
+          }
+    }
+    __syncthreads();
+    epilogue_rows<BT, 256>(d, bi, bj, Ct, ph * DT::HR, DT::HR, F16X3_PROD_SCALE);
+    __syncthreads();
+  }
+}
+
+// 256 x 256 tile on 8 waves (as sym_tile_dma8): wave w owns rows wr*128 ..
+// +128 and columns wc*64 .. +64, 4 x 2 accumulators.  Per slab (one 32-k
+// block): the 8 scaled FP8 MFMAs first, while the next slab's e4m3 fragments
+// and scales are read into the registers they free, then the two k-steps of
+// binary16 MFMAs, each followed by the next slab's hi fragments of that step.
+__device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int bj, char* smem) {
+  using DT = DmaTiling<FMT_F16MX, 8, false>;
+  constexpr int P = 2, BK = DT::BK, RB = DT::RB, S = DT::S;
+  constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES, SST = DT::STAGE_STRIDE;
+  constexpr int NT = 512;
+  constexpr int L = SB / (NT * 16);
+  constexpr int LT = L + 1;
+  static_assert(BK == 32 && SB % (NT * 16) == 0 && L >= 1, "one 32-k block per slab over 8 waves");
+  static_assert((S - 1) * LT <= 63, "vmcnt range");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 2, wc = wave & 3;
+  const int64_t ld = d.ld;
+  const int row0 = bi * 256, col0 = bj * 256;
+
+  constexpr int CPR = RB / 16, RPB = 1024 / RB;
+  const uint16_t* gsrc[L];
+  int lofs[L], loff[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int q = wave * L + i;
+    const int gp = q / DT::BLOCKS, blk = q % DT::BLOCKS;
+    const int g = gp / P, p = gp % P;
+    const void* bx = p == 0 ? d.x[0] : d.x[1];
+    const void* by = p == 0 ? d.y[0] : d.y[1];
+    const void* base = g == 0 ? bx : by;
+    gsrc[i] = (const uint16_t*)base + (int64_t)((g == 0 ? row0 : col0) + blk * RPB) * ld;
+    lofs[i] = gp * TB + blk * 1024;
+    const int r = blk * RPB + lane / CPR;
+    loff[i] = (lane / CPR) * (int)ld + ((lane % CPR) ^ dma_swz<RB>(r)) * 8;
+  }
+  const MxScaleLoad<256, 1, NT> sload(d, row0, col0, tid, lane);
+#define PSDP_MX8_ISSUE(KT, BUF)                                                                      \
+  {                                                                                                 \
+    _Pragma("unroll") for (int i = 0; i < L; ++i)                                                   \
+        __builtin_amdgcn_global_load_lds((const void*)(gsrc[i] + (KT) * BK + loff[i]),              \
+                                         (lds_ptr_t)(smem + (BUF) * SST + lofs[i]), 16, 0, 0);      \
+    sload.issue(KT, smem + (BUF) * SST + SB, ld);                                                   \
+  }
+
+  constexpr int MA = 4, MB = 2;
+  f32x16 acc[MA][MB];
+#pragma unroll
+  for (int a = 0; a < MA; ++a)
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
+
+  const int arow = (wr * 128 + (lane & 31)) * RB;
+  const int brow = (wc * 64 + (lane & 31)) * RB;
+  const int swzl = dma_swz<RB>(lane & 31);
+  const int h = lane >> 5;
+  const int KT = d.npad / BK;
+
+  u32x4 fa[2][MA], fb[2][MB];  // binary16 hi fragments per k-step
+  i32x8 fa8[MA], fb8[MB];      // e4m3 fragments of the slab's block
+  int sa[MA], sb[MB];
+  auto frag_a = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + p * TB + arow + m * 32 * RB + 16 * ((ks * 2 + h) ^ swzl));
+  };
+  auto frag_b = [&](const char* stg, int ks, int p, int m) -> u32x4 {
+    return *(const u32x4*)(stg + (P + p) * TB + brow + m * 32 * RB + 16 * ((ks * 2 + h) ^ swzl));
+  };
+  auto frag_a8 = [&](const char* stg, int m) -> i32x8 { return cat16(frag_a(stg, 0, 1, m), frag_a(stg, 1, 1, m)); };
+  auto frag_b8 = [&](const char* stg, int m) -> i32x8 { return cat16(frag_b(stg, 1, 1, m), frag_b(stg, 0, 1, m)); };
+  auto scl_a = [&](const char* stg, int m) -> int {
+    return *(const unsigned char*)(stg + SB + (wr * 128 + m * 32 + (lane & 31)) * 2 + h);
+  };
+  auto scl_b = [&](const char* stg, int m) -> int {
+    return *(const unsigned char*)(stg + SB + (256 + wc * 64 + m * 32 + (lane & 31)) * 2 + 1 - h);
+  };
+
+#pragma unroll
+  for (int st = 0; st < S; ++st) PSDP_MX8_ISSUE((st < KT ? st : KT - 1), st)
+  wait_vmcnt<(S - 1) * LT>();
+  lds_barrier();
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+    for (int m = 0; m < MA; ++m) fa[ks][m] = frag_a(smem, ks, 0, m);
+#pragma unroll
+    for (int m = 0; m < MB; ++m) fb[ks][m] = frag_b(smem, ks, 0, m);
+  }
+#pragma unroll
+  for (int m = 0; m < MA; ++m) fa8[m] = frag_a8(smem, m), sa[m] = scl_a(smem, m);
+#pragma unroll
+  for (int m = 0; m < MB; ++m) fb8[m] = frag_b8(smem, m), sb[m] = scl_b(smem, m);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+
+#define PSDP_MX8_CROSS()                                                                          \
+  _Pragma("unroll") for (int mj = 0; mj < MB; ++mj)                                               \
+  _Pragma("unroll") for (int mi = 0; mi < MA; ++mi)                                               \
+    acc[mi][mj] = mfma_mx(fa8[mi], fb8[mj], sa[mi], sb[mj], acc[mi][mj]);
+#define PSDP_MX8_HH(ks)                                                                           \
+  _Pragma("unroll") for (int mj = 0; mj < MB; ++mj)                                               \
+  _Pragma("unroll") for (int mi = 0; mi < MA; ++mi)                                               \
+    acc[mi][mj] = mfma32(fa[ks][mi], fb[ks][mj], acc[mi][mj], FMT_F16);
+
+  // reads per slab: 24 fragments + 6 scale bytes; MFMAs: 8 FP8 + 16 binary16
+  for (int kt = 0; kt + 1 < KT; ++kt) {
+    wait_vmcnt<(S - 2) * LT>();          // own DMA of slab kt + 1 landed
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // own reads of buffer kt % S complete
+    lds_barrier();
+    PSDP_MX8_ISSUE((kt + S < KT ? kt + S : KT - 1), kt % S)
+    const char* nxt = smem + ((kt + 1) % S) * SST;
+    PSDP_MX8_CROSS()
+#pragma unroll
+    for (int m = 0; m < MA; ++m) fa8[m] = frag_a8(nxt, m), sa[m] = scl_a(nxt, m);
+#pragma unroll
+    for (int m = 0; m < MB; ++m) fb8[m] = frag_b8(nxt, m), sb[m] = scl_b(nxt, m);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      PSDP_MX8_HH(ks)
+#pragma unroll
+      for (int m = 0; m < MA; ++m) fa[ks][m] = frag_a(nxt, ks, 0, m);
+#pragma unroll
+      for (int m = 0; m < MB; ++m) fb[ks][m] = frag_b(nxt, ks, 0, m);
+    }
+#pragma unroll
+    for (int g = 0; g < 24; ++g) {
+      if (g < LT) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);             // DS read
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);             // MFMA
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  PSDP_MX8_CROSS()
+  PSDP_MX8_HH(0)
+  PSDP_MX8_HH(1)
+#undef PSDP_MX8_CROSS
+#undef PSDP_MX8_HH
+#undef PSDP_MX8_ISSUE
+  __builtin_amdgcn_sched_barrier(0);
+  wait_vmcnt<0>();
+  lds_barrier();
+
+  float* Ct = (float*)smem;
+  constexpr int CS = 256 + 1;
+#define PSDP_STAGE8(PH)                                                                           \
+  _Pragma("unroll") for (int hh = 0; hh < 2; ++hh)                                                \
+  _Pragma("unroll") for (int mj = 0; mj < MB; ++mj)                                               \
+  _Pragma("unroll") for (int e = 0; e < 16; ++e) {                                                \
+    const int r = wr * 64 + hh * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);                   \
+    const int c = wc * 64 + mj * 32 + (lane & 31);                                                \
+    Ct[r * CS + c] = acc[2 * (PH) + hh][mj][e];                                                   \
+  }                                                                                               \
+  __syncthreads();                                                                                \
+  epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, F16X3_PROD_SCALE);                      \
+  epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, F16X3_PROD_SCALE);      \
+  __syncthreads();
+  PSDP_STAGE8(0)
+  PSDP_STAGE8(1)
+#undef PSDP_STAGE8
+}
+
+__global__ __launch_bounds__(512) void sym_prod_dma8_mx_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT_F16MX, 8, false>::LDS_BYTES];
+  const TileRef t = tiles[blockIdx.x];
+  if (t.bi < 0) return;
+  sym_tile_dma8_mx(d, t.bi, t.bj, smem);
+}
+
+template <int SUB>
+__global__ __launch_bounds__(256) void sym_prod_dma_mx_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT_F16MX, SUB, false>::LDS_BYTES];
+  const TileRef t = tiles[blockIdx.x];
+  if (t.bi < 0) return;
+  sym_tile_dma_mx<SUB>(d, t.bi, t.bj, smem);
+}
+
+template <int SUB>
+__global__ __launch_bounds__(256) void sym_prod_dma_mx_grouped_kernel(const ProdDesc* __restrict__ descs,
+                                                                      const TileRef* __restrict__ tiles,
+                                                                      int ntiles) {
+  __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT_F16MX, SUB, false>::LDS_BYTES];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileRef tr = tiles[t];
+    if (tr.mat < 0) continue;
+    sym_tile_dma_mx<SUB>(descs[tr.mat], tr.bi, tr.bj, smem);
+  }
+}
+
 template <int FMT, int SUB>
 __global__ __launch_bounds__(256) void sym_prod_dma_grouped_kernel(const ProdDesc* __restrict__ descs,
                                                                    const TileRef* __restrict__ tiles, int ntiles) {
@@ -1097,7 +1679,8 @@ static hipError_t launch_grouped(const ProdDesc* descs, const TileRef* tiles, in
 // 256 x 256 tiles (SUB = 8) exist for the 1- and 2-piece 16-bit formats with
 // plain (non-averaged) products.
 bool sym_big_tile_capable(int fmt, int avg) {
-  return !avg && (fmt == FMT_F16 || fmt == FMT_BF16 || fmt == FMT_F16X3 || fmt == FMT_BF16X3 || fmt == FMT_E4M3);
+  return !avg && (fmt == FMT_F16 || fmt == FMT_BF16 || fmt == FMT_F16X3 || fmt == FMT_BF16X3 || fmt == FMT_E4M3 ||
+                  fmt == FMT_F16MX);
 }
 
 // Producer waves (WS) are kept available but off: measured at n = 16384 they
@@ -1166,6 +1749,17 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
     PSDP_CASE2(FMT_F16X3)
     PSDP_CASE2(FMT_BF16X3)
     PSDP_CASEN(FMT_BF16X9)
+    case FMT_F16MX:
+      if (sub == 8) {
+        hipLaunchKernelGGL(sym_prod_dma8_mx_kernel, dim3(ntiles), dim3(512), 0, s, d, tiles);
+      } else if (sub == 4) {
+        hipLaunchKernelGGL((sym_prod_dma_mx_kernel<4>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+      } else if (sub == 2) {
+        hipLaunchKernelGGL((sym_prod_dma_mx_kernel<2>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+      } else {
+        return hipErrorInvalidValue;
+      }
+      return hipGetLastError();
     default:
       return hipErrorInvalidValue;
   }
@@ -1184,6 +1778,9 @@ hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc*
       case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 4>(descs, tiles, ntiles, grid, s);
       case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 4>(descs, tiles, ntiles, grid, s);
       case FMT_E4M3: return launch_dma_grouped<FMT_E4M3, 4>(descs, tiles, ntiles, grid, s);
+      case FMT_F16MX:
+        hipLaunchKernelGGL((sym_prod_dma_mx_grouped_kernel<4>), dim3(grid), dim3(256), 0, s, descs, tiles, ntiles);
+        return hipGetLastError();
       default: break;  // FP32 and averaged products: the 64 x 64 kernels below
     }
   }
@@ -1210,6 +1807,9 @@ hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc*
     case FMT_BF16X3: return launch_dma_grouped<FMT_BF16X3, 2>(descs, tiles, ntiles, grid, s);
     case FMT_BF16X9: return launch_dma_grouped<FMT_BF16X9, 2>(descs, tiles, ntiles, grid, s);
     case FMT_E4M3: return launch_dma_grouped<FMT_E4M3, 2>(descs, tiles, ntiles, grid, s);
+    case FMT_F16MX:
+      hipLaunchKernelGGL((sym_prod_dma_mx_grouped_kernel<2>), dim3(grid), dim3(256), 0, s, descs, tiles, ntiles);
+      return hipGetLastError();
     default: return hipErrorInvalidValue;
   }
 }

@@ -44,6 +44,7 @@ METHODS = {
     "bf16x3": (dict(method="custom", table="fp32", step_format=["bf16x3"] * 10, final_format="bf16x3"),
                ["bf16x3"]),
     "composite_FP32_emulated": (dict(method="composite_FP32_emulated"), ["bf16x9"]),
+    "mixed_mx": (dict(method="mixed_mx"), ["f16mx", "f16"]),
 }
 
 
@@ -51,6 +52,8 @@ def oracle_chain(Xs, name):
     c32, c16 = O.fp32_chain_coefficients(), O.fp16_chain_coefficients()
     if name == "mixed":
         return O.composite_chain(Xs, c16, ["f16x3"] * 4 + ["f16"] * 3, "f16", sym_mode="mirror")
+    if name == "mixed_mx":
+        return O.composite_chain(Xs, c16, ["f16mx"] * 4 + ["f16"] * 3, "f16", sym_mode="mirror")
     if name == "composite_FP32":
         return O.composite_chain(Xs, c32, ["f32"] * 10, "f32", sym_mode="mirror")
     if name == "bf16x3":
@@ -89,12 +92,13 @@ def test_big_tiles_projection_vs_oracle(handle, name):
     e_or = O.rel_fro_error(oracle_chain(X32.astype(np.float64) / up, name) * up, Pt)
     print(f"  gpu {err:.3e} oracle {e_or:.3e}", flush=True)
     assert 0.7 * e_or - 2e-6 <= err <= 1.3 * e_or + 2e-6
-    if name == "mixed":
+    if name in ("mixed", "mixed_mx"):
         assert err <= 1e-3
 
 
 @pytest.mark.parametrize("n", [4100, 6000])
-@pytest.mark.parametrize("name,bar", [("mixed", 1e-3), ("bf16x3", 1e-4), ("composite_FP32", 1e-4)])
+@pytest.mark.parametrize("name,bar", [("mixed", 1e-3), ("mixed_mx", 1e-3), ("bf16x3", 1e-4),
+                                      ("composite_FP32", 1e-4)])
 def test_big_tiles_projection_bitwise(handle, n, name, bar):
     # n = 4100: padded to 4352 = 17 x 256 (153 tiles, padding rows inside the
     # last tile row); n = 6000: 24 x 24 tiles = 300, more than one round on
@@ -115,7 +119,8 @@ def test_big_tiles_projection_bitwise(handle, n, name, bar):
 
 
 # ------------------------------------------------------------------ full sizes
-@pytest.mark.parametrize("method,bar", [("composite_FP32", 1e-4), ("composite_FP16", 5e-3), ("mixed", 1e-3)])
+@pytest.mark.parametrize("method,bar", [("composite_FP32", 1e-4), ("composite_FP16", 5e-3), ("mixed", 1e-3),
+                                        ("mixed_mx", 1e-3)])
 def test_config2_full_size(handle, method, bar):
     # n = 8192, lambda ~ N(0,1) scaled to max 10 with 30% in [-1e-2, 1e-2].
     # The reference's FP16 chain is at 2-3e-3 on this spectrum (above 1e-3,
@@ -135,14 +140,15 @@ def test_config2_full_size(handle, method, bar):
         assert err >= 1e-3  # the reference FP16 chain misses 1e-3 on this spectrum
 
 
-def test_config3_full_size_mixed(handle):
+@pytest.mark.parametrize("method", ["mixed", "mixed_mx"])
+def test_config3_full_size(handle, method):
     n = 16384
-    print(f"config 3 mixed n={n}", flush=True)
+    print(f"config 3 {method} n={n}", flush=True)
     X, Pm, _ = W.spectral_matrix(n, 3, "cfg3", device=DEV)
     Xd = X.to(torch.float32)
     del X
     handle.set_seed(3)
-    Y = P.project(handle, Xd, method="mixed")
+    Y = P.project(handle, Xd, method=method)
     del Xd
     err = W.rel_fro_error(Y, Pm)
     print(f"  rel err {err:.3e}", flush=True)

@@ -388,6 +388,91 @@ def test_e4m3_product_error_band():
     assert 5e-3 < e < 6e-2, e
 
 
+def test_mx_e4m3_known_answers():
+    # block max 1.0 (exponent field 127) -> scale 2^-7: scaled values up to 128,
+    # so values on the e4m3 grid times 2^-7 come back unchanged
+    blk = np.array([1.0, 0.5, -0.25, 0.75, 2.0 ** -9, 0.0] + [0.125] * 26, dtype=np.float32)
+    assert np.array_equal(O.mx_e4m3(blk[None, :], 1)[0], blk)
+    # max 1.99 keeps scale 2^-7 (scaled max 254.7 -> 256, no saturation at 448)
+    b2 = np.zeros(32, np.float32)
+    b2[0], b2[1] = 1.99, 1.0 + 1.0 / 16  # 1 + 1/16 -> 136 scaled: quantum 16 -> 128 (tie to even) -> 1.0
+    got = O.mx_e4m3(b2[None, :], 1)[0]
+    assert got[0] == 2.0 and got[1] == 1.0
+    # an all-zero block and a block of one tiny value
+    assert not O.mx_e4m3(np.zeros((3, 32), np.float32), 1).any()
+    t = np.zeros(32, np.float32)
+    t[5] = 3e-30
+    g = O.mx_e4m3(t[None, :], 1)[0]
+    assert g[5] == pytest.approx(3e-30, rel=2 ** -4) and not np.delete(g, 5).any()
+
+
+def test_mx_e4m3_brute_force():
+    torch = pytest.importorskip("torch")
+    codes = torch.arange(256, dtype=torch.uint8).view(torch.float8_e4m3fn).float().numpy().astype(np.float64)
+    grid = np.unique(codes[np.isfinite(codes)])
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((40, 96)) * np.exp2(rng.integers(-20, 8, (40, 1)))).astype(np.float32)
+    x[3, 40:70] = 0.0
+    got = O.mx_e4m3(x, 1).astype(np.float64)
+    for r in range(x.shape[0]):
+        for k0 in range(0, 96, 32):
+            blk = x[r, k0:k0 + 32].astype(np.float64)
+            m = np.abs(blk).max()
+            # scale 2^(floor(log2 m) - 7) from frexp (m = f 2^e, f in [0.5, 1))
+            sc = 1.0 if m == 0 else 2.0 ** (np.frexp(m)[1] - 1 - 7)
+            assert m / sc < 256
+            for k in range(32):
+                v = blk[k] / sc
+                cand = grid[np.abs(grid - v) == np.abs(grid - v).min()]
+                if len(cand) > 1:  # tie: the even code (last mantissa bit 0)
+                    cand = [c for c in cand if (torch.tensor(c, dtype=torch.float32).to(torch.float8_e4m3fn)
+                                                .view(torch.uint8).item() & 1) == 0]
+                assert got[r, k0 + k] == cand[0] * sc
+    # blocks along axis 0 are blocks along axis 1 of the transpose; a ragged end is zero padded
+    assert np.array_equal(O.mx_e4m3(x, 0), O.mx_e4m3(x.T, 1).T)
+    xr = x[:, :70]
+    assert np.array_equal(O.mx_e4m3(xr, 1), O.mx_e4m3(np.pad(xr, ((0, 0), (0, 26))), 1)[:, :70])
+
+
+def test_f16mx_product_error_band():
+    # hi.hi exact in float32 plus cross terms from e4m3 copies (relative
+    # error 2^-4 of terms 2^-11 below hi.hi): ~1e-5, far below binary16
+    # (~3e-4) and above FP16x3 (~3e-7).  Planted mistakes -- cross terms
+    # dropped, paired hi8.hi8 / lo8.lo8, or blocked along the wrong axis of
+    # a non-symmetric operand -- leave the band.
+    X, _, _ = O.spectral_input(256, 5, "uniform")
+    Y, _, lam = O.spectral_input(256, 9, "normal")
+    Y = Y / np.abs(lam).max()
+    X32, Y32 = X.astype(np.float32), Y.astype(np.float32)
+    exact = X32.astype(np.float64) @ Y32.astype(np.float64)
+    rel = lambda c: np.linalg.norm(c - exact) / np.linalg.norm(exact)
+    e_mx = rel(O.emulated_product(X32, Y32, "f16mx"))
+    e16, e3 = rel(O.emulated_product(X32, Y32, "f16")), rel(O.emulated_product(X32, Y32, "f16x3"))
+    assert 3e-6 < e_mx < 3e-5 and e_mx < e16 / 10 and e_mx > 10 * e3, (e_mx, e16, e3)
+    px, py = O.mx_split(X32, 1), O.mx_split(Y32, 0)
+    s16 = np.float32(1.0 / 65536)
+    mm = lambda a, b: (a @ b).astype(np.float32)
+    assert rel(mm(px[0], py[0]) * s16) > e16 / 4  # no cross terms
+    assert rel((mm(px[1], py[1]) + mm(px[0], py[0])) * s16) > 0.5  # hi8.hi8 instead of hi8.lo8
+    Z = np.triu(Y32) * np.float32(3)  # non-symmetric: the k axis of y is axis 0
+    pz_wrong = O.mx_split(Z, 1)
+    ez = np.linalg.norm(O.emulated_product(X32, Z, "f16mx") - X32.astype(np.float64) @ Z) / np.linalg.norm(
+        X32.astype(np.float64) @ Z)
+    ew = np.linalg.norm((mm(px[1], pz_wrong[2]) + mm(px[2], pz_wrong[1]) + mm(px[0], pz_wrong[0])) * s16
+                        - X32.astype(np.float64) @ Z) / np.linalg.norm(X32.astype(np.float64) @ Z)
+    assert ez < 3e-5 and ew > ez
+
+
+def test_mixed_mx_chain_error():
+    # the mixed_mx schedule keeps the 1e-3 bar, a little above mixed
+    X, Q, lam = O.spectral_input(512, 3, "uniform")
+    Pt = O.closed_form_projection(Q, lam)
+    c = O.fp16_chain_coefficients()
+    e_mx = O.rel_fro_error(O.composite_chain(X, c, ["f16mx"] * 4 + ["f16"] * 3, "f16", sym_mode="mirror"), Pt)
+    e_m = O.rel_fro_error(O.composite_chain(X, c, ["f16x3"] * 4 + ["f16"] * 3, "f16", sym_mode="mirror"), Pt)
+    assert e_mx <= 1e-3 and e_m <= e_mx <= 1.3 * e_m, (e_mx, e_m)
+
+
 @pytest.mark.parametrize("kind", ["normal", "uniform"])
 def test_eig_fp64_staged_matches_eigh(kind):
     # The timed CPU baseline (dsytrd + dstedc + dormtr, the stages of dsyevd)
