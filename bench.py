@@ -5,7 +5,9 @@ Workload (config 3 of the benchmark set): one symmetric n=16384 matrix
 X = Q diag(lam) Q^T stored in float32, lam ~ U[-5,5] with 10% of the
 eigenvalues in [-1e-3, 1e-3], Haar Q.  A step is one full projection
 (Lanczos bound + composite polynomial chain + output) with the shipped
-mixed-precision schedule; inputs are resident in HBM before timing.
+mixed-precision schedule (mixed_mx: FP16 coefficient table, steps 0-3 as
+binary16 hi.hi plus scaled-e4m3 cross terms, binary16 after); inputs are
+resident in HBM before timing.
 Reported next to it: the plain FP32 composite chain on the same input (the
 speed target is >= 5x at <= 1e-3 relative Frobenius error vs the exact
 projection), the dominant kernel's roofline fraction, and a CPU baseline
@@ -23,8 +25,17 @@ import time
 
 METRIC = "n=16384 projection ms + batched proj/s at ≤1e-3 rel err vs FP64 EVD; p99 lat"
 PEAK_TFLOPS = {"f32": 157.3, "f16": 2516.6, "f16x3": 2516.6, "bf16": 2516.6, "bf16x3": 2516.6,
-               "bf16x9": 2516.6}
-PRODUCTS = {"f32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3, "bf16x9": 9}
+               "bf16x9": 2516.6, "f16mx": 2516.6}
+PEAK_FP8_TFLOPS = 5033.2
+# MFMA products per symmetric product, in binary16-MFMA equivalents: f16mx is
+# one binary16 product (hi.hi) plus two e4m3 cross products on the scaled FP8
+# MFMA, which runs at twice the binary16 rate (W + 2W/2 = 2W at the binary16 peak)
+PRODUCTS = {"f32": 1, "f16": 1, "f16x3": 3, "bf16": 1, "bf16x3": 3, "bf16x9": 9, "f16mx": 2}
+# operand formats of the shipped FP16-table schedules: 7 steps of 3 products
+# (steps 0-3 split, 4-6 binary16) and the final binary16 product
+SCHEDULE_FMTS = {"mixed": ["f16x3"] * 4 + ["f16"] * 3, "mixed_mx": ["f16mx"] * 4 + ["f16"] * 3}
+KERNEL_LABEL = {"f16x3": "sym_prod_dma8_kernel, 8 waves", "bf16x3": "sym_prod_dma8_kernel, 8 waves",
+                "f16mx": "sym_prod_dma8_mx_kernel, 8 waves: binary16 hi.hi + scaled e4m3 cross terms"}
 
 
 def log(*a):
@@ -149,12 +160,31 @@ def precision_study(args, P, W, torch, h, X, Y, Pm, out):
                                       "binary16 products, not of FP8 MFMA (no FP8 kernel is built: the "
                                       "schedule diverges numerically)")
         log(f"precision study: {name}: {res[-1]['ms']} ms, rel err {err:.3e}")
-    res.append({"variant": "shipped mixed (fp16 table, FP16x3 steps 0-3, binary16 rest)", "ms": out["value"],
-                "rel_err": out["rel_err"], "meets_1e-3": bool(out["rel_err"] <= 1e-3)})
+    if args.method != "mixed":
+        cfg = P.default_config("mixed", synchronize=0)
+        if args.autotune:
+            h.autotune(X.shape[0], config=cfg)
+        ms = _timed(P, torch, h, X, Y, cfg, 3)
+        err = W.rel_fro_error(Y, Pm)
+        res.append({"variant": "fp16 table, FP16x3 steps 0-3, binary16 rest (mixed)", "ms": round(ms, 3),
+                    "rel_err": err, "meets_1e-3": bool(err <= 1e-3)})
+        log(f"precision study: mixed: {ms:.2f} ms, rel err {err:.3e}")
+    res.append({"variant": f"shipped {args.method}" + (" (fp16 table, F16MX steps 0-3: binary16 hi.hi + scaled "
+                                                        "e4m3 cross terms, binary16 rest)" if args.method == "mixed_mx"
+                                                        else ""),
+                "ms": out["value"], "rel_err": out["rel_err"], "meets_1e-3": bool(out["rel_err"] <= 1e-3)})
     return res
 
 
-MIXED_UNITS = 12 * 3 + 10  # mixed schedule: 12 FP16x3 products (3 MFMA products each) + 10 binary16
+def chain_mfma_ms(method, sizes):
+    """(binary16-equivalent flops, ms at the MFMA peaks) of one projection of
+    each size in sizes under an FP16-table schedule: 7 steps of 3 products and
+    the final binary16 product, W_sym(n) = n^2 (n + 1) flops per MFMA product."""
+    fmts = [f for f in SCHEDULE_FMTS[method] for _ in range(3)] + ["f16"]
+    w = sum(float(n) * n * (n + 1) for n in sizes)
+    f16 = sum({"f16": 1, "f16x3": 3, "f16mx": 1}[f] for f in fmts) * w
+    fp8 = sum(2 if f == "f16mx" else 0 for f in fmts) * w
+    return f16 + fp8 / 2, (f16 / (PEAK_TFLOPS["f16"] * 1e12) + fp8 / (PEAK_FP8_TFLOPS * 1e12)) * 1e3
 
 
 def bench_batched(args, P, W, torch):
@@ -188,7 +218,7 @@ def bench_batched(args, P, W, torch):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     errs = [W.rel_fro_error(y, pm) for y, pm in zip(outs, ps)]
-    flops = MIXED_UNITS * sum(float(n) * n * (n + 1) for n in sizes) if args.method == "mixed" else None
+    flops, t_mfma = chain_mfma_ms(args.method, sizes) if args.method in SCHEDULE_FMTS else (None, None)
     res = {"workload": f"{len(sizes)} blocks, n log-uniform [32,1024] seed 4 (sum n = {sum(sizes)}), "
                        f"lambda U[-1,1], {args.method}; one grouped launch per product, replayed from a HIP graph",
            "ms_per_batch": round(ms, 3), "proj_per_s": round(len(sizes) / (ms * 1e-3), 1),
@@ -204,10 +234,10 @@ def bench_batched(args, P, W, torch):
         npad2 = sum(float(((n + 63) // 64 * 64) ** 2) for n in sizes)
         lanczos_bytes = 41.0 * sum(symv_bytes((n + 63) // 64 * 64) for n in sizes)
         per_step = {"f16x3": 4 + (4 + 4) + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
-        steps = ["f16x3"] * 4 + ["f16"] * 3
+        per_step["f16mx"] = per_step["f16x3"]  # binary16 hi + e4m3 hi and lo bytes (+ 1/16 B of scales)
+        steps = SCHEDULE_FMTS[args.method]
         bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + lanczos_bytes
         t_hbm = bytes_batch / 8.0e12 * 1e3
-        t_mfma = flops / (PEAK_TFLOPS["f16"] * 1e12) * 1e3
         res["roofline"] = {"bound": "hbm" if t_hbm > t_mfma else "mfma", "model_ms_hbm": round(t_hbm, 3),
                            "model_ms_mfma": round(t_mfma, 3),
                            "frac_of_bound": round(max(t_hbm, t_mfma) / ms, 4),
@@ -320,7 +350,7 @@ def bench_small_configs(args, P, W, torch):
     Y = torch.empty_like(X)
     res = {"workload": "n=8192, Haar Q (seed 2), lambda N(0,1) scaled to max|lambda|=10, 30% in [-1e-2,1e-2], "
                        "float32 input"}
-    for m in ("composite_FP32", "composite_FP16", "mixed"):
+    for m in ("composite_FP32", "composite_FP16", "mixed", "mixed_mx"):
         cfg = P.default_config(m, synchronize=0)
         if args.autotune:
             h.autotune(8192, config=cfg)
@@ -329,6 +359,7 @@ def bench_small_configs(args, P, W, torch):
         res[m] = {"ms": round(ms, 3), "rel_err": err, "meets_1e-3": bool(err <= 1e-3)}
     res["fp16_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["composite_FP16"]["ms"], 2)
     res["mixed_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["mixed"]["ms"], 2)
+    res["mixed_mx_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["mixed_mx"]["ms"], 2)
     out["cfg2_n8192"] = res
     log(f"config 2: {res}")
     del X, Y, Pm
@@ -404,7 +435,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--method", default="mixed")
+    ap.add_argument("--method", default="mixed_mx")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="CPU baseline block size (the whole matrix by default; 0: skip)")
@@ -503,12 +534,14 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[fmt], 4),
                 "traffic": measured_traffic(n, fmt),
                 "kernel": f"symmetric product, {fmt} operands ("
-                          f"{'sym_prod_dma8_kernel, 8 waves' if fmt in ('f16x3', 'bf16x3') else 'sym_prod_dma_kernel, 4 waves'}"
+                          f"{KERNEL_LABEL.get(fmt, 'sym_prod_dma_kernel, 4 waves')}"
                           f" on 256x256 tiles; autotuned tiles "
                           f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds + the last partial "
                           "round as 64x64 tiles in a second launch)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
-                "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)"}
+                "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)" +
+                              (" in binary16-MFMA equivalents: hi.hi on the binary16 MFMA + 2 e4m3 cross "
+                               "products on the scaled FP8 MFMA at twice the rate" if fmt == "f16mx" else "")}
     total_chain_flops = sum(PRODUCTS[f] * wsym * c for f, (c, _) in prods.items()) / args.steps
     chain_ms = sum(t for _, t in prods.values()) / args.steps
 
@@ -516,10 +549,14 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f16" if args.method in ("mixed", "composite_FP16") else
-           ("f32" if args.method == "composite_FP32" else args.method),
-           "dtype_note": "binary16 MFMA operands with f32 accumulation; steps 0-3 use a 3-term hi/lo "
-                         "binary16 split (FP32-level products); Lanczos in f64"
-           if args.method == "mixed" else "",
+           ("f16+e4m3" if args.method == "mixed_mx" else
+            ("f32" if args.method == "composite_FP32" else args.method)),
+           "dtype_note": {"mixed": "binary16 MFMA operands with f32 accumulation; steps 0-3 use a 3-term hi/lo "
+                                   "binary16 split (FP32-level products); Lanczos in f64",
+                          "mixed_mx": "f32 accumulation throughout; steps 0-3: hi = binary16 of 2^8 v on the "
+                                      "binary16 MFMA plus the hi.lo and lo.hi cross terms as e4m3 with one "
+                                      "power-of-two scale per 32 k (scaled FP8 MFMA); steps 4-6 and the final "
+                                      "product binary16; Lanczos in f64"}.get(args.method, ""),
            "data": ("synthetic: Haar Q (QR of a seeded Gaussian, torch FP64 on the GPU)" if args.basis == "haar"
                     else "synthetic: Householder Q (one seeded reflector)") +
                    ", lambda ~ U[-5,5] with 10% in [-1e-3,1e-3]; float32 storage",

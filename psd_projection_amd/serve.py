@@ -51,6 +51,7 @@ class Request:
     t_done: float = 0.0
     err: Any = None             # optional device scalar (relative error), set by a callback
     released: bool = False
+    joined_with: int = -1       # session requests still in flight when this one joined (small requests)
 
     @property
     def latency(self) -> float:
@@ -132,6 +133,9 @@ class TorchExecutor:
 
     def session_active(self) -> bool:
         return bool(self.tickets)
+
+    def session_blocks(self) -> int:
+        return len(self.tickets)
 
     # ------------------------------------------------------------------ lanes
     def launch(self, lane: int, r: Request):
@@ -257,6 +261,7 @@ class ProjectionServer:
             if self.held_bytes() + r.bytes > self.budget:
                 break
             if r.n <= self.small_max:
+                r.joined_with = self.ex.session_blocks()
                 if not self.ex.session_add(r):
                     self.refusals += 1
                     break

@@ -57,7 +57,7 @@ def test_served_results_equal_alone_results(arrivals):
     assert st["p99_ms"] >= st["p50_ms"] > 0
     if arrivals == "mid_batch":
         # the small requests that arrived mid-chain joined the running session:
-        # admitted before the first wave's small requests had finished
-        early = [r for r in reqs[:7] if r.n <= 1024]
+        # at least one was admitted while other session requests were still on
+        # the GPU (counted by the server at admission, not by wall clock)
         late = [r for r in reqs[7:] if r.n <= 1024]
-        assert min(r.t_launch for r in late) < max(r.t_done for r in early)
+        assert max(r.joined_with for r in late) > 0, [(r.rid, r.n, r.joined_with) for r in late]

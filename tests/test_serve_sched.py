@@ -72,6 +72,9 @@ class FakeExecutor:
     def session_active(self):
         return bool(self.blocks)
 
+    def session_blocks(self):
+        return len(self.blocks)
+
     def alloc_large(self, r):
         if self.fail_allocs:
             self.fail_allocs -= 1
@@ -163,6 +166,7 @@ def test_late_request_joins_at_the_next_product():
     assert b.t_done - a.t_done == pytest.approx(first_b - ticks_a[0], abs=5e-5)
     assert first_b - ticks_a[0] < 12e-3
     assert len(done) == 2
+    assert (a.joined_with, b.joined_with) == (0, 1)      # B was admitted with A in flight
 
 
 def test_outputs_count_until_released_and_admission_is_fifo():
