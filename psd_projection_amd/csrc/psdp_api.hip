@@ -523,15 +523,44 @@ static int max_sub(int64_t n, int fmt, int avg) {
   return choose_sub(n);
 }
 
-// Built-in tile choice (psdp_autotune can override it per size).  Measured
-// on MI355X for the 16-bit direct-to-LDS kernels: 256 x 256 tiles win from
-// n = 8192 (n = 8192: 32.4 ms per mixed projection against 32.5 ms), below
-// that too few of them fill the 256 CUs and 64 x 64 tiles win (n = 2048:
-// 1.66 ms against 4.35 ms with 256 x 256; n = 4096: 6.58 against 7.14 ms).
-// Other formats keep 128 x 128 above n = 1024.
-static int launch_sub(int64_t n, int fmt, int avg) {
+// Built-in tile choice (psdp_autotune can override it per size) for the
+// 16-bit direct-to-LDS kernels: the tile size with the smallest modelled
+// launch cost, (rounds of tiles over the device's workgroup slots) x (time
+// of one round relative to a round of 64 x 64 tiles: 1.65 for 128 x 128,
+// 4.3 for 256 x 256, each over the same K).  64 x 64 tiles run two
+// workgroups per CU, the larger ones one; a partial last round of 256 x 256
+// tiles that fills at most half the CUs is re-cut into 64 x 64 tiles
+// (get_split_tiles).  The weights come from timing every tile size on
+// mixed and mixed_mx projections (ms, 64 / 128 / 256; all results bitwise
+// equal): n = 2560: 2.16 / 2.00 / 3.58, 3072: 3.02 / 3.23 / 4.00, 3584:
+// 4.30 / 3.75 / 4.51, 4096: 5.79 / 5.57 / 5.08, 5120: 10.1 / 9.0 / 6.9,
+// 6144: 16.8 / 13.8 / 12.0, 7168: 25.8 / 21.3 / 17.0, 8192: 41.4 / 31.5 /
+// 23.6; the model picks the fastest at every one of them.  Other formats
+// keep 128 x 128 above n = 1024.
+static int launch_sub(int64_t n, int fmt, int avg, int cus) {
   if (!sym_big_tile_capable(fmt, avg)) return choose_sub(n);
-  return n >= 8192 ? 8 : 2;
+  const int top = max_sub(n, fmt, avg);
+  const int64_t btop = 32 * top, npad = (n + btop - 1) / btop * btop;
+  const int64_t slots = std::max(cus, 1);
+  auto tiles = [&](int64_t bt) {
+    const int64_t T = npad / bt;
+    return T * (T + 1) / 2;
+  };
+  auto rounds = [](int64_t t, int64_t per) { return (double)((t + per - 1) / per); };
+  int sub = 2;
+  double best = rounds(tiles(64), 2 * slots);
+  if (top >= 4) {
+    const double c = 1.65 * rounds(tiles(128), slots);
+    if (c < best) best = c, sub = 4;
+  }
+  if (top >= 8) {
+    const int64_t total = tiles(256), full = total / slots * slots, rem = total - full;
+    const double c = (full == 0 || rem == 0 || 2 * rem > slots)
+                         ? 4.3 * rounds(total, slots)
+                         : 4.3 * (double)(full / slots) + rounds(16 * rem, 2 * slots);
+    if (c < best) best = c, sub = 8;
+  }
+  return sub;
 }
 
 static psdp_status_t make_plan(const psdp_config_t* cfg, int64_t n, Plan& p) {
@@ -665,7 +694,7 @@ static psdp_status_t launch_product(psdp_handle_t h, const Plan& p, int fmt, int
 static int tuned_sub(psdp_handle_t h, const Plan& p, int fmt, int avg) {
   auto it = h->tuned.find(std::make_tuple(p.n, fmt, avg));
   if (it != h->tuned.end() && it->second <= max_sub(p.n, fmt, avg)) return it->second;
-  return launch_sub(p.n, fmt, avg);
+  return launch_sub(p.n, fmt, avg, h->cus);
 }
 
 // fixed_scale: s of the entry points without the bound (1 for the reference's
@@ -1039,7 +1068,7 @@ psdp_status_t psdp_autotune(psdp_handle_t h, int64_t n, const psdp_config_t* cfg
     for (int q = 0; q < 3; ++q) d.outp[q] = b.p2p[q];
     DESCCHK(d, fmt);
     const int top = max_sub(p.n, fmt, avg);
-    int best = launch_sub(p.n, fmt, avg);
+    int best = launch_sub(p.n, fmt, avg, h->cus);
     float best_ms = 1e30f;
     for (int sub = top; sub >= 2; sub /= 2) {
       STCHK(launch_product(h, p, fmt, avg, d, sub, s));  // warm-up (tile lists, caches)
@@ -1276,7 +1305,7 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   const int avg = average ? 1 : 0;
   const int top = max_sub(n, format, avg);
   auto it = h->tuned.find(std::make_tuple((int)n, format, avg));
-  const int sub = it != h->tuned.end() && it->second <= top ? it->second : launch_sub(n, format, avg);
+  const int sub = it != h->tuned.end() && it->second <= top ? it->second : launch_sub(n, format, avg, h->cus);
   const int bt = 32 * sub, btop = 32 * top;
   const int npad = (int)((n + btop - 1) / btop * btop), ld = npad + LD_PAD;
   const int64_t N = (int64_t)npad * ld;

@@ -1,6 +1,6 @@
 """GPU parity of the kernels behind the large-n projections.
 
-The built-in tile choice runs 256 x 256 tiles only from n = 8192, where the
+The built-in tile choice runs 256 x 256 tiles from n = 4096, where the
 oracle's emulation is too slow to compare with; here the 256 x 256 tiles are
 forced at smaller n (psdp_set_tuned_tile), so that full projections run the
 store, polynomial and (I + A)/2 epilogues of the 256 x 256 kernels (the
