@@ -260,6 +260,14 @@ psdp_status_t psdp_session_step(psdp_session_t session, int* in_flight_ticks);
 psdp_status_t psdp_session_poll(psdp_session_t session, int64_t* tickets, int max, int* count, int* active,
                                 int* in_flight_ticks);
 
+/* on != 0 (the default): a product on 256 x 256 tiles whose last round fills
+ * at most half the CUs splits that round's tiles' K range into chunks that run
+ * side by side and are added in chunk order before the epilogue (no
+ * counterpart in the reference: a tiling of this library; off, the round is
+ * re-cut into 64 x 64 tiles).  Split tiles' entries differ from the whole-K
+ * order in the last bits; with 0 every tile size gives bitwise the same
+ * result. */
+psdp_status_t psdp_set_split_k(psdp_handle_t h, int on);
 /* Tile classes of the batched call: blocks with n >= big_block_n use
  * 128 x 128 tiles, the rest 64 x 64 (0: the built-in default); concurrent != 0
  * runs the two classes of each product on two streams. */

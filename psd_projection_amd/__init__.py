@@ -154,6 +154,7 @@ _PROTOS = {
     "psdp_eig_fp32_psd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "psdp_set_eigenpair_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "psdp_set_batched_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "psdp_set_split_k": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "psdp_set_batched_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "psdp_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(Config)]),
     "psdp_tuned_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
@@ -299,6 +300,13 @@ class Handle:
         or "last"; normalize divides the kept Ritz vectors by their norms."""
         m = {"first": RITZ_RESIDUAL_FIRST_ROW, "last": RITZ_RESIDUAL_LAST_ROW}.get(residual_row, residual_row)
         _check(lib().psdp_set_eigenpair_options(self._h, int(m), 1 if normalize else 0))
+
+    def set_split_k(self, on=True):
+        """Split products (default on): a last round of 256 x 256 tiles that
+        fills at most half the CUs runs its tiles' K range in chunks side by
+        side; off, it is re-cut into 64 x 64 tiles and every tile size gives
+        bitwise the same result."""
+        _check(lib().psdp_set_split_k(self._h, 1 if on else 0))
 
     def set_batched_tiling(self, big_block_n=0, concurrent=False):
         """Blocks with n >= big_block_n run on 128 x 128 tiles in batched calls

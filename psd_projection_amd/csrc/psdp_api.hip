@@ -32,6 +32,7 @@ namespace psdp {
 hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, const TileRef* tiles, int ntiles,
                               hipStream_t s);
 bool sym_big_tile_capable(int fmt, int avg);
+hipError_t launch_split_reduce(int fmt, const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s);
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,
                                       int ntiles, int grid, hipStream_t s);
 hipError_t launch_sym_pad(const void* X, int x_f64, int64_t xld, int n, float* M, int64_t ld, int npad,
@@ -223,6 +224,14 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// Launch lists of one product on 256 x 256 tiles (T x T of them, upper
+// triangle): whole-K tiles, a re-cut 64 x 64 tail, and the K chunks of split
+// tiles with the list their reduction runs over.
+struct BigTiles {
+  TileList big, small, split, reduce;
+  int ksplit = 0, kchunk64 = 0;
+};
+
 struct psdp_handle_st {
   bool profiling = false;
   std::vector<ProfRec> prof;
@@ -259,7 +268,10 @@ struct psdp_handle_st {
   hipGraphExec_t bgraph = nullptr;
   std::vector<int64_t> bgraph_key;
   std::map<std::pair<int, int>, TileList> tiles;  // (tiles per dim, tile size)
-  std::map<int, std::pair<TileList, TileList>> split_tiles;  // T -> (big rounds, TAIL_SUB tail)
+  std::map<std::pair<int, int>, struct BigTiles> big_tiles;  // (T, split products on) -> launch lists
+  int split_k = 1;               // split products for under-filled rounds of 256 x 256 tiles (psdp_set_split_k)
+  float* part = nullptr;         // partial tiles of split products
+  size_t part_bytes = 0;
   int cus = 256;                                  // compute units of the device
   void* scratch = nullptr;                        // this device's scratch array (psdp_project_f32_noalloc)
   size_t scratch_bytes = 0;
@@ -352,48 +364,136 @@ static psdp_status_t get_tiles(psdp_handle_t h, int T, int bt, TileList& tl) {
   return PSDP_OK;
 }
 
-// 256 x 256 tiles in whole rounds over the device's CUs, plus the last,
-// partial round re-cut into 64 x 64 tiles for a second launch: one big tile
-// per CU at a time, so a partial round of big tiles (2080 tiles = 8.1 rounds
-// at n = 16384 on 256 CUs) would cost a whole round.  Re-cut, the 32 leftover
-// tiles become 512 small ones that occupy every CU (two workgroups each);
-// measured at n = 16384 a binary16x3 product takes 12.14 ms with the 64 x 64
-// tail against 12.37 ms with a 128 x 128 tail (which fills only half the
-// CUs).  Only when the partial round is at most half the CUs.  Every launch
-// of the tail list uses TAIL_SUB.
+// How a product on T x T tiles of 256 x 256 fills the device (cus CUs, one
+// 256 x 256 workgroup each).  Whole rounds run whole-K tiles.  A last round
+// that fills at most half the CUs is either re-cut into 64 x 64 tiles (two
+// workgroups per CU, split products off; measured at n = 16384 a binary16x3
+// product takes 12.14 ms with the 64 x 64 tail against 12.37 ms with a
+// 128 x 128 tail) or, by default, split: each of its tiles runs as S K
+// chunks (S = cus / tiles in the round, at most 16) whose partial sums a
+// reduction adds in chunk order before the epilogue.  Measured per mixed_mx
+// projection (ms, re-cut / split): n = 6144 12.15 / 11.32, 8192 23.79 /
+// 23.84, 10240 42.0 / 40.4, 16384 156.1 / 154.8.  Splitting every tile of a
+// launch with fewer tiles than CUs (S = 2-4) was slower (n = 4096: 5.09
+// against 5.29 ms; per chunk the prologue, the partial tile's store and its
+// reduction cost more than the idle CUs), so such launches run whole-K tiles.
+// Split tiles sum their K in S pieces, so their entries differ from the
+// whole-K order in the last bits (every other tiling agrees bitwise).  Chunks
+// are whole multiples of 64 k, none empty.
+struct SplitShape {
+  int total = 0, full = 0, rem = 0;
+  int nsplit = 0, S = 1, chunk64 = 0;  // split tiles (the last nsplit of the order), chunks each
+  bool recut = false;                  // 64 x 64 tail instead (split products off)
+};
+static SplitShape split_shape(int T, int cus, bool split_on) {
+  SplitShape z;
+  const int ncu = std::max(cus, 1);
+  z.total = T * (T + 1) / 2;
+  z.full = z.total / ncu * ncu;
+  z.rem = z.total - z.full;
+  const int K64 = 4 * T;  // 64-element units of K (npad = 256 T)
+  if (split_on && z.full > 0 && z.rem > 0 && 2 * z.rem <= ncu) {
+    z.S = std::min(std::min(ncu / z.rem, 16), K64 / 2);
+    if (z.S > 1) z.nsplit = z.rem;
+  }
+  if (z.nsplit) {
+    z.chunk64 = (K64 + z.S - 1) / z.S;
+    z.S = (K64 + z.chunk64 - 1) / z.chunk64;
+  } else {
+    z.S = 1;
+  }
+  z.recut = !z.nsplit && !(z.full == 0 || z.rem == 0 || 2 * z.rem > ncu);
+  return z;
+}
+
 constexpr int TAIL_SUB = 2;  // 64 x 64
-static psdp_status_t get_split_tiles(psdp_handle_t h, int T, TileList& big, TileList& small) {
-  auto it = h->split_tiles.find(T);
-  if (it != h->split_tiles.end()) {
-    big = it->second.first;
-    small = it->second.second;
+static psdp_status_t get_big_tiles(psdp_handle_t h, int T, const BigTiles** out) {
+  const auto key = std::make_pair(T, h->split_k ? 1 : 0);
+  auto it = h->big_tiles.find(key);
+  if (it != h->big_tiles.end()) {
+    *out = &it->second;
     return PSDP_OK;
   }
   const std::vector<TileRef> ord = super_tile_order(T, 8);
-  const int total = (int)ord.size(), ncu = std::max(h->cus, 1);
-  const int full = total / ncu * ncu, rem = total - full;
-  std::vector<TileRef> bigv, smallv;
-  if (full == 0 || rem == 0 || 2 * rem > ncu) {
-    bigv = xcd_interleave(ord);
-  } else {
-    bigv = xcd_interleave(std::vector<TileRef>(ord.begin(), ord.begin() + full));
+  const SplitShape z = split_shape(T, h->cus, h->split_k != 0);
+  const int total = (int)ord.size();
+  std::vector<TileRef> bigv, smallv, splitv, redv;
+  if (z.recut) {
+    bigv = xcd_interleave(std::vector<TileRef>(ord.begin(), ord.begin() + z.full));
     std::vector<TileRef> tail;
     constexpr int K = 8 / TAIL_SUB;  // tail tiles per big tile edge
-    for (int t = full; t < total; ++t)
+    for (int t = z.full; t < total; ++t)
       for (int a = 0; a < K; ++a)
         for (int b = 0; b < K; ++b) {
           const int r = K * ord[t].bi + a, c = K * ord[t].bj + b;
           if (r <= c) tail.push_back(TileRef{0, r, c, 0});  // the lower part of a diagonal tile is a mirror
         }
     smallv = xcd_interleave(tail);
+  } else {
+    const int nbig = total - z.nsplit;
+    if (nbig) bigv = xcd_interleave(std::vector<TileRef>(ord.begin(), ord.begin() + nbig));
+    std::vector<TileRef> chunks;
+    for (int s = 0; s < (z.nsplit ? z.S : 0); ++s)
+      for (int j = 0; j < z.nsplit; ++j) chunks.push_back(TileRef{j, ord[nbig + j].bi, ord[nbig + j].bj, s});
+    if (!chunks.empty()) splitv = xcd_interleave(chunks);
+    for (int j = 0; j < z.nsplit; ++j) redv.push_back(TileRef{j, ord[nbig + j].bi, ord[nbig + j].bj, 0});
   }
-  TileList nb, ns;
-  STCHK(upload_tiles(bigv, nb));
-  STCHK(upload_tiles(smallv, ns));
-  if (smallv.empty()) ns.count = 0;
-  h->split_tiles[T] = std::make_pair(nb, ns);
-  big = nb;
-  small = ns;
+  BigTiles bt;
+  auto up = [&](const std::vector<TileRef>& v, TileList& l) -> psdp_status_t {
+    if (v.empty()) return PSDP_OK;
+    return upload_tiles(v, l);
+  };
+  STCHK(up(bigv, bt.big));
+  STCHK(up(smallv, bt.small));
+  STCHK(up(splitv, bt.split));
+  STCHK(up(redv, bt.reduce));
+  bt.ksplit = z.nsplit ? z.S : 0;
+  bt.kchunk64 = z.chunk64;
+  auto& slot = h->big_tiles[key];
+  slot = bt;
+  *out = &slot;
+  return PSDP_OK;
+}
+
+// Device memory for the partial tiles of split products (grown on demand;
+// the products of one stream use it one after another).
+static psdp_status_t part_reserve(psdp_handle_t h, size_t bytes, float** out) {
+  if (bytes > h->part_bytes) {
+    if (h->part) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipFree(h->part));
+      h->part = nullptr;
+      h->part_bytes = 0;
+    }
+    HIPCHK(hipMalloc((void**)&h->part, bytes));
+    h->part_bytes = bytes;
+  }
+  *out = h->part;
+  return PSDP_OK;
+}
+
+// One product on 256 x 256 tiles: whole-K tiles, then the re-cut tail or the
+// split tiles' chunks and their reduction (which runs the epilogue).
+static psdp_status_t launch_big_product(psdp_handle_t h, int T, int fmt, int avg, const ProdDesc& d, hipStream_t s) {
+  const BigTiles* bt = nullptr;
+  STCHK(get_big_tiles(h, T, &bt));
+  if (bt->big.count) HIPCHK(launch_sym_product(fmt, 8, avg, d, bt->big.dev, bt->big.count, s));
+  if (bt->small.count) HIPCHK(launch_sym_product(fmt, TAIL_SUB, avg, d, bt->small.dev, bt->small.count, s));
+  if (bt->split.count) {
+    float* part = nullptr;
+    STCHK(part_reserve(h, (size_t)bt->reduce.count * bt->ksplit * 65536 * sizeof(float), &part));
+    ProdDesc dp = d;
+    dp.mode = EPI_PARTIAL;
+    dp.part = part;
+    dp.ksplit = bt->ksplit;
+    dp.kchunk64 = bt->kchunk64;
+    HIPCHK(launch_sym_product(fmt, 8, avg, dp, bt->split.dev, bt->split.count, s));
+    ProdDesc dr = d;
+    dr.part = part;
+    dr.ksplit = bt->ksplit;
+    dr.kchunk64 = bt->kchunk64;
+    HIPCHK(launch_split_reduce(fmt, dr, bt->reduce.dev, bt->reduce.count, s));
+  }
   return PSDP_OK;
 }
 
@@ -675,13 +775,7 @@ static psdp_status_t launch_product(psdp_handle_t h, const Plan& p, int fmt, int
     HIPCHK(launch_sym_product(fmt, sub, avg, d, ft->list[sub], ft->count[sub], s));
     return PSDP_OK;
   }
-  if (sub == 8) {
-    TileList big, small;
-    STCHK(get_split_tiles(h, p.npad / 256, big, small));
-    HIPCHK(launch_sym_product(fmt, 8, avg, d, big.dev, big.count, s));
-    if (small.count) HIPCHK(launch_sym_product(fmt, TAIL_SUB, avg, d, small.dev, small.count, s));
-    return PSDP_OK;
-  }
+  if (sub == 8) return launch_big_product(h, p.npad / 256, fmt, avg, d, s);
   TileList tl;
   STCHK(get_tiles(h, p.npad / (32 * sub), 32 * sub, tl));
   HIPCHK(launch_sym_product(fmt, sub, avg, d, tl.dev, tl.count, s));
@@ -937,10 +1031,10 @@ psdp_status_t psdp_destroy(psdp_handle_t h) {
   (void)hipStreamSynchronize(h->stream);
   clear_profile(h);
   for (auto& kv : h->tiles) (void)hipFree(kv.second.dev);
-  for (auto& kv : h->split_tiles) {
-    (void)hipFree(kv.second.first.dev);
-    (void)hipFree(kv.second.second.dev);
-  }
+  for (auto& kv : h->big_tiles)
+    for (TileList* l : {&kv.second.big, &kv.second.small, &kv.second.split, &kv.second.reduce})
+      if (l->dev) (void)hipFree(l->dev);
+  if (h->part) (void)hipFree(h->part);
   for (int k = 0; k < psdp_handle_st::NSTAGE; ++k) {
     if (h->stage_ev[k]) (void)hipEventSynchronize(h->stage_ev[k]);
     if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
@@ -1001,6 +1095,13 @@ psdp_status_t psdp_set_batched_graph(psdp_handle_t h, int on) {
   if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
   std::lock_guard<std::mutex> lock(h->mu);
   h->batch_graph = on != 0;
+  return PSDP_OK;
+}
+
+psdp_status_t psdp_set_split_k(psdp_handle_t h, int on) {
+  if (!h) return fail(PSDP_ERR_INVALID_ARG, "null handle");
+  std::lock_guard<std::mutex> lock(h->mu);
+  h->split_k = on != 0;
   return PSDP_OK;
 }
 
@@ -1322,9 +1423,8 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
     py[i] = (uint16_t*)(c + 2 * f32b + (3 + i) * u16b);
   }
   hipStream_t s = h->stream;
-  TileList tl, tail;
-  if (sub == 8) STCHK(get_split_tiles(h, npad / bt, tl, tail));  // the tiling the projections use
-  else STCHK(get_tiles(h, npad / bt, bt, tl));
+  TileList tl;
+  if (sub != 8) STCHK(get_tiles(h, npad / bt, bt, tl));
   {
     ProfScope ps(h, PSDP_PROF_PREP);
     HIPCHK(launch_sym_pad(X, 0, n, (int)n, mx, ld, npad, s));
@@ -1347,8 +1447,8 @@ psdp_status_t psdp_sym_product_f32(psdp_handle_t h, const float* X, const float*
   d.scale = nullptr;
   {
     ProfScope ps(h, PSDP_PROF_PROD_F32 + format);
-    HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
-    if (tail.count) HIPCHK(launch_sym_product(format, TAIL_SUB, average ? 1 : 0, d, tail.dev, tail.count, s));
+    if (sub == 8) STCHK(launch_big_product(h, npad / bt, format, average ? 1 : 0, d, s));  // as the projections
+    else HIPCHK(launch_sym_product(format, sub, average ? 1 : 0, d, tl.dev, tl.count, s));
   }
   HIPCHK(hipStreamSynchronize(s));
   return PSDP_OK;

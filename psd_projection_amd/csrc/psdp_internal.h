@@ -44,8 +44,15 @@ enum : int {
   EPI_STORE = 0,       // C = X Y   -> c32 (optional) + operand pieces (out_fmt)
   EPI_POLY = 1,        // A = a A + b P3 + c X Y (in place) + pieces of A
   EPI_POLY_HALF_ID = 2,// as EPI_POLY, then S = (I + A)/2 written as pieces / c32
-  EPI_OUTPUT = 3       // out = scale * X Y on the logical n x n part (f64 or f32)
+  EPI_OUTPUT = 3,      // out = scale * X Y on the logical n x n part (f64 or f32)
+  EPI_PARTIAL = 4      // raw accumulators of one K chunk of a 256 x 256 tile -> part (split products)
 };
+
+// Split products (EPI_PARTIAL): K is cut into ksplit chunks of kchunk64 * 64
+// elements (the last one shorter); tile list entry {slot, bi, bj, s} sums
+// chunk s of tile slot from zero and stores the 256 x 256 raw accumulators,
+// row-major, at part + (slot * ksplit + s) * 65536.  A second kernel adds
+// the chunks of each tile in index order and runs the product's epilogue.
 
 // One symmetric n_pad x n_pad product C = X Y of two symmetric matrices held
 // as operand pieces (row-major view, leading dimension ld, zero padded).
@@ -66,6 +73,9 @@ struct ProdDesc {
   int out_f64;       // 1: double output, 0: float output
   const double* scale;  // EPI_OUTPUT multiplier (device scalar); null -> 1
   int* nonfinite;    // EPI_OUTPUT: set to 1 when a written value is not finite (null: unchecked)
+  float* part;       // split products: partial tiles (EPI_PARTIAL writes, the reduction reads)
+  int ksplit;        // split products: K chunks per tile
+  int kchunk64;      // split products: chunk length in units of 64 k
 };
 
 // Tile of the upper triangle: matrix index + block coordinates (bi <= bj).

@@ -388,6 +388,22 @@ __device__ __forceinline__ void epilogue_store_rows_mx(float* C, int r0, int NR,
 
 // MXOUT = false leaves out the FMT_F16MX store path (kernels that never write
 // it: the float32 and averaged products, whose registers it would crowd).
+// EPI_PARTIAL (split products): the staged rows (local rows 0..NR-1 of C,
+// row stride BT + 1), unscaled, to rows r0 .. r0 + NR of the chunk's partial
+// tile (row-major BT x BT floats).
+template <int BT, int NT>
+__device__ __forceinline__ void partial_rows(float* ptile, const float* C, int r0, int NR) {
+  constexpr int CS = BT + 1;
+  constexpr int Q4 = BT / 4;
+  auto dst = gbl(ptile);
+  for (int q = threadIdx.x; q < NR * Q4; q += NT) {
+    const int r = q / Q4, c0 = (q % Q4) * 4;
+    const float* src = C + r * CS + c0;
+    *(__attribute__((address_space(1))) f32x4*)(dst + (int64_t)(r0 + r) * BT + c0) =
+        f32x4{src[0], src[1], src[2], src[3]};
+  }
+}
+
 template <int BT, int NT = 256, int GMAX = 8, bool MXOUT = true>
 __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
                                               float cscale = 1.0f) {
@@ -760,14 +776,33 @@ __device__ __forceinline__ void lds_barrier() {
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+// Slabs [k0, k0 + KT) of a tile: the whole K, or for a split product
+// (EPI_PARTIAL) chunk s of d.ksplit, each d.kchunk64 * 64 elements long (the
+// host sizes the chunks so that none is empty).
+template <int BK>
+__device__ __forceinline__ int tile_k_range(const ProdDesc& d, int s, int& k0) {
+  const int KT = d.npad / BK;
+  if (d.mode != EPI_PARTIAL) {
+    k0 = 0;
+    return KT;
+  }
+  constexpr int PER = 64 / BK;  // slabs per 64 k
+  k0 = s * d.kchunk64 * PER;
+  return min(KT - k0, d.kchunk64 * PER);
+}
+
 // WS (producer waves): the workgroup has 8 waves; waves 0-3 compute (one per
 // SIMD) and never issue a global->LDS load, waves 4-7 only issue them and
 // wait for them.  A wave that issues a 1 KB global_load_lds stalls its own
 // instruction stream for many cycles, and with one compute wave per SIMD
 // those cycles are lost to the matrix cores; with a producer wave beside it
 // the compute wave keeps issuing MFMAs.  Requires <= 256 registers per wave.
-template <int FMT, int SUB, bool WS>
-__device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, char* smem) {
+// PART: the chunk of a split product (EPI_PARTIAL) whose epilogue only copies
+// the raw tile out (a separate instantiation, so that the register budget of
+// the full epilogue is untouched).
+template <int FMT, int SUB, bool WS, bool PART = false>
+__device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, char* smem, int split_slot = 0,
+                                             int split_chunk = 0) {
   using DT = DmaTiling<FMT, SUB, WS>;
   constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, S = DT::S, L = DT::L;
   constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
@@ -823,7 +858,8 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   const int arow = (wr * DT::WT + (lane & 31)) * RB;
   const int brow = (wc * DT::WT + (lane & 31)) * RB;
   const int swzl = dma_swz<RB>(lane & 31);
-  const int KT = d.npad / BK;
+  int KT = d.npad / BK, k0 = 0;  // slabs k0 .. k0 + KT (a chunk of K for a split product)
+  if constexpr (PART) KT = tile_k_range<BK>(d, split_chunk, k0);
 
   // fa/fb hold the slab being multiplied; na receives the A fragments of the
   // next slab, and the B fragments of the next slab overwrite fb column block
@@ -845,7 +881,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   // uniform and the loop body is one block that the scheduler can interleave.
   if (loader) {
 #pragma unroll
-    for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((st < KT ? st : KT - 1) * BK, st) }
+    for (int st = 0; st < S; ++st) { PSDP_DMA_ISSUE((k0 + (st < KT ? st : KT - 1)) * BK, st) }
     wait_vmcnt<(S - 1) * L>();
   }
   lds_barrier();
@@ -921,7 +957,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     if (loader) wait_vmcnt<(S - 2) * L>();
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     lds_barrier();
-    if (loader) { PSDP_DMA_ISSUE((kt + S < KT ? kt + S : KT - 1) * BK, kt % S) }
+    if (loader) { PSDP_DMA_ISSUE((k0 + (kt + S < KT ? kt + S : KT - 1)) * BK, kt % S) }
     if (!compute) continue;
     const char* nxt = smem + ((kt + 1) % S) * SB;
 #pragma unroll
@@ -970,6 +1006,8 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
   if (loader) wait_vmcnt<0>();  // trailing stage copies land before the epilogue reuses LDS
 #undef PSDP_DMA_ISSUE
   lds_barrier();  // every wave is done reading the stages before they hold C
+  float* ptile = nullptr;  // split products: this chunk's partial tile
+  if constexpr (PART) ptile = d.part + ((int64_t)split_slot * d.ksplit + split_chunk) * 65536;
 
   // ---------------------------------------------------------------- epilogue
   constexpr float PSCALE = (FMT == FMT_F16X3 || FMT == FMT_E4M3) ? F16X3_PROD_SCALE : 1.0f;
@@ -991,17 +1029,18 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
           }
     }
     __syncthreads();
-    epilogue_rows<DT::BT, NT>(d, bi, bj, Ct, ph * DT::HR, DT::HR, PSCALE);
+    if constexpr (PART) partial_rows<DT::BT, NT>(ptile, Ct, ph * DT::HR, DT::HR);
+    else epilogue_rows<DT::BT, NT>(d, bi, bj, Ct, ph * DT::HR, DT::HR, PSCALE);
     __syncthreads();
   }
 }
 
-template <int FMT, int SUB, bool WS>
+template <int FMT, int SUB, bool WS, bool PART = false>
 __global__ __launch_bounds__(WS ? 512 : 256) void sym_prod_dma_kernel(ProdDesc d, const TileRef* __restrict__ tiles) {
   __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, SUB, WS>::LDS_BYTES];
   const TileRef t = tiles[blockIdx.x];
   if (t.bi < 0) return;
-  sym_tile_dma<FMT, SUB, WS>(d, t.bi, t.bj, smem);
+  sym_tile_dma<FMT, SUB, WS, PART>(d, t.bi, t.bj, smem, t.mat, t.pad);
 }
 
 // 256 x 256 tile with 8 waves (two per SIMD): wave w owns rows wr*128 .. +128
@@ -1014,7 +1053,8 @@ __global__ __launch_bounds__(WS ? 512 : 256) void sym_prod_dma_kernel(ProdDesc d
 // slab replace them k-step by k-step once that k-step's MFMAs are issued.
 // The epilogue preloads at most 2 quads per thread (256 registers per wave).
 template <int FMT>
-__device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, int bi, int bj, char* smem) {
+__device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, const TileRef& t, char* smem) {
+  const int bi = t.bi, bj = t.bj;
   using DT = DmaTiling<FMT, 8, false>;
   constexpr int P = DT::P, BK = DT::BK, RB = DT::RB, S = DT::S;
   constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES;
@@ -1061,7 +1101,10 @@ __device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, int bi, int bj,
   const int arow = (wr * 128 + (lane & 31)) * RB;
   const int brow = (wc * 64 + (lane & 31)) * RB;
   const int swzl = dma_swz<RB>(lane & 31);
-  const int KT = d.npad / BK;
+  int k0;
+  const int KT = tile_k_range<BK>(d, t.pad, k0);
+#pragma unroll
+  for (int i = 0; i < L; ++i) gsrc[i] += (int64_t)k0 * BK;
 
   u32x4 fa[KS][P][MA], fb[KS][P][MB];
   auto frag_a = [&](const char* stg, int ks, int p, int m) -> u32x4 {
@@ -1132,6 +1175,7 @@ __device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, int bi, int bj,
   __builtin_amdgcn_sched_barrier(0);
   wait_vmcnt<0>();
   lds_barrier();
+  float* const ptile = d.mode == EPI_PARTIAL ? d.part + ((int64_t)t.mat * d.ksplit + t.pad) * 65536 : nullptr;
 
   constexpr float PSCALE = (FMT == FMT_F16X3 || FMT == FMT_E4M3) ? F16X3_PROD_SCALE : 1.0f;
   // Two phases; in phase ph every wave stages its row blocks 2ph and 2ph+1,
@@ -1149,8 +1193,13 @@ __device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, int bi, int bj,
     Ct[r * CS + c] = acc[2 * (PH) + h][mj][e];                                                    \
   }                                                                                               \
   __syncthreads();                                                                                \
-  epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, PSCALE);                                \
-  epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, PSCALE);                \
+  if (ptile) {                                                                                    \
+    partial_rows<256, NT>(ptile, Ct, (PH) * 64, 64);                                              \
+    partial_rows<256, NT>(ptile, Ct + 64 * CS, 128 + (PH) * 64, 64);                              \
+  } else {                                                                                        \
+    epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, PSCALE);                              \
+    epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, PSCALE);              \
+  }                                                                                               \
   __syncthreads();
   PSDP_STAGE8(0)
   PSDP_STAGE8(1)
@@ -1162,7 +1211,7 @@ __global__ __launch_bounds__(512) void sym_prod_dma8_kernel(ProdDesc d, const Ti
   __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT, 8, false>::LDS_BYTES];
   const TileRef t = tiles[blockIdx.x];
   if (t.bi < 0) return;
-  sym_tile_dma8<FMT>(d, t.bi, t.bj, smem);
+  sym_tile_dma8<FMT>(d, t, smem);
 }
 
 // ------------------------------------------------------------------ FMT_F16MX
@@ -1409,7 +1458,8 @@ __device__ __forceinline__ void sym_tile_dma_mx(const ProdDesc& d, int bi, int b
 // block): the 8 scaled FP8 MFMAs first, while the next slab's e4m3 fragments
 // and scales are read into the registers they free, then the two k-steps of
 // binary16 MFMAs, each followed by the next slab's hi fragments of that step.
-__device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int bj, char* smem) {
+__device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, const TileRef& t, char* smem) {
+  const int bi = t.bi, bj = t.bj;
   using DT = DmaTiling<FMT_F16MX, 8, false>;
   constexpr int P = 2, BK = DT::BK, RB = DT::RB, S = DT::S;
   constexpr int TB = DT::TILE_BYTES, SB = DT::STAGE_BYTES, SST = DT::STAGE_STRIDE;
@@ -1440,7 +1490,7 @@ __device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int 
     const int r = blk * RPB + lane / CPR;
     loff[i] = (lane / CPR) * (int)ld + ((lane % CPR) ^ dma_swz<RB>(r)) * 8;
   }
-  const MxScaleLoad<256, 1, NT> sload(d, row0, col0, tid, lane);
+  MxScaleLoad<256, 1, NT> sload(d, row0, col0, tid, lane);
 #define PSDP_MX8_ISSUE(KT, BUF)                                                                      \
   {                                                                                                 \
     _Pragma("unroll") for (int i = 0; i < L; ++i)                                                   \
@@ -1462,7 +1512,11 @@ __device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int 
   const int brow = (wc * 64 + (lane & 31)) * RB;
   const int swzl = dma_swz<RB>(lane & 31);
   const int h = lane >> 5;
-  const int KT = d.npad / BK;
+  int k0;
+  const int KT = tile_k_range<BK>(d, t.pad, k0);
+#pragma unroll
+  for (int i = 0; i < L; ++i) gsrc[i] += (int64_t)k0 * BK;
+  sload.src += (int64_t)k0 * ld;  // one scale row per 32-k block
 
   u32x4 fa[2][MA], fb[2][MB];  // binary16 hi fragments per k-step
   i32x8 fa8[MA], fb8[MB];      // e4m3 fragments of the slab's block
@@ -1545,6 +1599,7 @@ __device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int 
   __builtin_amdgcn_sched_barrier(0);
   wait_vmcnt<0>();
   lds_barrier();
+  float* const ptile = d.mode == EPI_PARTIAL ? d.part + ((int64_t)t.mat * d.ksplit + t.pad) * 65536 : nullptr;
 
   float* Ct = (float*)smem;
   constexpr int CS = 256 + 1;
@@ -1557,8 +1612,13 @@ __device__ __forceinline__ void sym_tile_dma8_mx(const ProdDesc& d, int bi, int 
     Ct[r * CS + c] = acc[2 * (PH) + hh][mj][e];                                                   \
   }                                                                                               \
   __syncthreads();                                                                                \
-  epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, F16X3_PROD_SCALE);                      \
-  epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, F16X3_PROD_SCALE);      \
+  if (ptile) {                                                                                    \
+    partial_rows<256, NT>(ptile, Ct, (PH) * 64, 64);                                              \
+    partial_rows<256, NT>(ptile, Ct + 64 * CS, 128 + (PH) * 64, 64);                              \
+  } else {                                                                                        \
+    epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, F16X3_PROD_SCALE);                    \
+    epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, F16X3_PROD_SCALE);    \
+  }                                                                                               \
   __syncthreads();
   PSDP_STAGE8(0)
   PSDP_STAGE8(1)
@@ -1569,7 +1629,69 @@ __global__ __launch_bounds__(512) void sym_prod_dma8_mx_kernel(ProdDesc d, const
   __shared__ __attribute__((aligned(16))) char smem[DmaTiling<FMT_F16MX, 8, false>::LDS_BYTES];
   const TileRef t = tiles[blockIdx.x];
   if (t.bi < 0) return;
-  sym_tile_dma8_mx(d, t.bi, t.bj, smem);
+  sym_tile_dma8_mx(d, t, smem);
+}
+
+// Second half of a split product: four workgroups per split 256 x 256 tile
+// ({slot, bi, bj, -}), one per 64-row quarter, add the tile's d.ksplit partial
+// tiles in chunk order (((p0 + p1) + p2) + ...), scale the sum by cscale (the
+// 2^-16 of FP16x3 / F16MX products, inside the epilogue) and run the product's
+// epilogue (mode d.mode) on the quarter.  A thread sums 4 quads at a time and
+// keeps 4 chunks of loads in flight (clamped addresses past the last chunk,
+// whose values are not added), so the reads do not wait one by one.
+__global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const TileRef* __restrict__ tiles,
+                                                               float cscale) {
+  __shared__ __attribute__((aligned(16))) float Ct[64 * 257];
+  const TileRef t = tiles[blockIdx.x >> 2];
+  const int qtr = blockIdx.x & 3;
+  if (t.bi < 0) return;
+  const int S = d.ksplit;
+  auto src = gbl((const float*)d.part) + (int64_t)t.mat * S * 65536 + qtr * 64 * 256;
+  constexpr int G = 4, KB = 4;
+  typedef __attribute__((address_space(1))) const f32x4 gf4;
+  for (int q0 = threadIdx.x; q0 < 64 * 64; q0 += 256 * G) {
+    f32x4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g * 256;
+      acc[g] = *(gf4*)(src + (q >> 6) * 256 + (q & 63) * 4);
+    }
+    for (int k0 = 1; k0 < S; k0 += KB) {
+      f32x4 w[KB][G];
+#pragma unroll
+      for (int kk = 0; kk < KB; ++kk) {
+        const int k = min(k0 + kk, S - 1);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int q = q0 + g * 256;
+          w[kk][g] = *(gf4*)(src + (int64_t)k * 65536 + (q >> 6) * 256 + (q & 63) * 4);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < KB; ++kk) {
+        if (k0 + kk < S) {
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            acc[g].x += w[kk][g].x;
+            acc[g].y += w[kk][g].y;
+            acc[g].z += w[kk][g].z;
+            acc[g].w += w[kk][g].w;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int q = q0 + g * 256;
+      float* row = Ct + (q >> 6) * 257 + (q & 63) * 4;
+      row[0] = acc[g].x;
+      row[1] = acc[g].y;
+      row[2] = acc[g].z;
+      row[3] = acc[g].w;
+    }
+  }
+  __syncthreads();
+  epilogue_rows<256, 256>(d, t.bi, t.bj, Ct, qtr * 64, 64, cscale);
 }
 
 template <int SUB>
@@ -1699,6 +1821,12 @@ static hipError_t launch_dma(const ProdDesc& d, const TileRef* tiles, int ntiles
     return hipGetLastError();
   }
   constexpr bool WS = false;
+  if constexpr (SUB == 8) {
+    if (d.mode == EPI_PARTIAL) {
+      hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, 8, WS, true>), dim3(ntiles), dim3(256), 0, s, d, tiles);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((sym_prod_dma_kernel<FMT, SUB, WS>), dim3(ntiles), dim3(WS ? 512 : 256), 0, s, d, tiles);
   return hipGetLastError();
 }
@@ -1766,6 +1894,12 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
 #undef PSDP_CASE1
 #undef PSDP_CASEN
 #undef PSDP_CASE2
+}
+
+hipError_t launch_split_reduce(int fmt, const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
+  const float cscale = (fmt == FMT_F16X3 || fmt == FMT_E4M3 || fmt == FMT_F16MX) ? F16X3_PROD_SCALE : 1.0f;
+  hipLaunchKernelGGL(sym_split_reduce_kernel, dim3(4 * ntiles), dim3(256), 0, s, d, tiles, cscale);
+  return hipGetLastError();
 }
 
 hipError_t launch_sym_product_grouped(int fmt, int sub, int avg, const ProdDesc* descs, const TileRef* tiles,

@@ -102,20 +102,65 @@ def test_big_tiles_projection_vs_oracle(handle, name):
 def test_big_tiles_projection_bitwise(handle, n, name, bar):
     # n = 4100: padded to 4352 = 17 x 256 (153 tiles, padding rows inside the
     # last tile row); n = 6000: 24 x 24 tiles = 300, more than one round on
-    # 256 CUs, so the last partial round runs as 64 x 64 tiles in a second
-    # launch of the same product.
+    # 256 CUs, so the last partial round (44 tiles) runs either as 64 x 64
+    # tiles in a second launch (split products off: bitwise the 64 x 64
+    # tiling) or, by default, as 5 K chunks per tile plus their reduction,
+    # whose epilogues (store, polynomial, (I + A)/2, output) must meet the
+    # same bar.
     seed = 8
     print(f"big-tile projection {name} n={n}", flush=True)
     X, Pm, _ = W.spectral_matrix(n, 40 + n % 7, "cfg3", device=DEV)
     Xd = X.to(torch.float32)
     del X
-    Y256 = project_tiled(handle, Xd, name, n, 256, seed)
+    handle.set_split_k(False)
+    try:
+        Y256 = project_tiled(handle, Xd, name, n, 256, seed)
+    finally:
+        handle.set_split_k(True)
     Y64 = project_tiled(handle, Xd, name, n, 64, seed)
     assert torch.equal(Y256, Y64)
     assert torch.equal(Y256, Y256.T)
     err = W.rel_fro_error(Y256, Pm)
     print(f"  gpu rel err {err:.3e}", flush=True)
     assert err <= bar
+    Ys = project_tiled(handle, Xd, name, n, 256, seed)
+    assert torch.equal(Ys, Ys.T)
+    err_s = W.rel_fro_error(Ys, Pm)
+    print(f"  split products: rel err {err_s:.3e}", flush=True)
+    assert err_s <= bar
+    if n == 4100:  # one round of 153 tiles: no split
+        assert torch.equal(Ys, Y256)
+
+
+@pytest.mark.parametrize("fmt", ["f16", "f16x3", "f16mx", "bf16x3", "e4m3"])
+@pytest.mark.parametrize("n", [5700, 6144])
+def test_split_products_match_whole_k(handle, n, fmt):
+    # n = 5700: 23 x 23 tiles of 256 = 276, last round 20 tiles in 12 K
+    # chunks; n = 6144: 300 tiles, last round 44 tiles in 5 chunks.  A split
+    # tile sums its K in chunks and adds them in order: the same product to
+    # float32 rounding, with every kernel of the 256 x 256 tiles (4-wave
+    # one-piece, 8-wave two-piece, 8-wave F16MX) and the reduction's output
+    # epilogue.
+    g = torch.Generator(device=DEV).manual_seed(n + len(fmt))
+    X = torch.randn(n, n, device=DEV, generator=g) / n ** 0.5
+    X = (X + X.T) * 0.5
+    Y = torch.randn(n, n, device=DEV, generator=g) / n ** 0.5
+    Y = (Y + Y.T) * 0.5
+    print(f"split product {fmt} n={n}", flush=True)
+    handle.set_tile(n, fmt, 256)
+    try:
+        handle.set_split_k(False)
+        C0 = P.sym_product(handle, X, Y, fmt)
+        handle.set_split_k(True)
+        C1 = P.sym_product(handle, X, Y, fmt)
+    finally:
+        handle.set_split_k(True)
+        handle.set_tile(n, fmt, 0)
+    torch.cuda.synchronize()
+    d = float((C1.double() - C0.double()).norm() / C0.double().norm())
+    print(f"  split vs whole K: {d:.2e}", flush=True)
+    assert 0 < d <= 2e-6
+    assert torch.equal(C1, C1.T)
 
 
 # ------------------------------------------------------------------ full sizes

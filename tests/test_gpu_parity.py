@@ -61,26 +61,35 @@ def test_sym_product(handle, fmt, avg, n):
 @pytest.mark.parametrize("fmt", ["f16", "f16x3", "bf16", "bf16x3"])
 def test_sym_product_split_tail(handle, fmt):
     # n = 6144: 300 tiles of 256 x 256, i.e. one round on 256 CUs plus a partial
-    # one that runs as 64 x 64 tiles in a second launch (on devices with
-    # fewer CUs the split differs; the result must not).  The two-piece
-    # formats run their 256 x 256 tiles on the 8-wave kernel, the one-piece
-    # formats on the 4-wave kernel.
-    # The built-in choice at this size is 64 x 64 tiles; 256 x 256 is set by
-    # hand, and both must give the same bits.
+    # one of 44 tiles.  With split products off that round runs as 64 x 64
+    # tiles in a second launch and the result is bitwise the 64 x 64 tiling's
+    # (on devices with fewer CUs the rounds differ; the result must not); by
+    # default its tiles run as 5 K chunks plus a reduction, within float32
+    # rounding of the oracle's product.  The two-piece formats run their
+    # 256 x 256 tiles on the 8-wave kernel, the one-piece formats on the
+    # 4-wave kernel.
     n = 6144
     print(f"sym_product split tail fmt={fmt} n={n}", flush=True)
     X, _, _ = O.spectral_input(n, 77, "uniform")
     X32 = X.astype(np.float32)
     Xd = to_dev(X32, torch.float32)
-    handle.set_tile(n, fmt, 256)
-    C = from_dev(P.sym_product(handle, Xd, Xd, fmt))
-    handle.set_tile(n, fmt, 0)
-    C64 = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+    try:
+        handle.set_tile(n, fmt, 256)
+        Cs = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+        handle.set_split_k(False)
+        C = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+        handle.set_tile(n, fmt, 64)
+        C64 = from_dev(P.sym_product(handle, Xd, Xd, fmt))
+    finally:
+        handle.set_split_k(True)
+        handle.set_tile(n, fmt, 0)
     assert np.array_equal(C, C64)
     ref = O.sym_product(X32, X32, fmt, "mirror").astype(np.float64)
-    assert np.array_equal(C, C.T)
-    err = np.linalg.norm(C - ref) / np.linalg.norm(ref)
-    assert err <= 2e-6, err
+    for M in (C, Cs):
+        assert np.array_equal(M, M.T)
+        err = np.linalg.norm(M - ref) / np.linalg.norm(ref)
+        assert err <= 2e-6, err
+    assert not np.array_equal(Cs, C)  # the split path ran
 
 
 def test_sym_product_average_rejected_for_split_formats(handle):

@@ -45,8 +45,8 @@ def test_served_results_equal_alone_results(arrivals):
     for r, Pm in zip(reqs, ps):
         if r.n <= 1024:
             alone = P.project_batched(h, [r.X], config=cfg)[0]
-        else:
-            alone = P.project(h, r.X, config=cfg)
+        else:  # alone on the lane's own handle (same seed, and the tile sizes its autotuner chose)
+            alone = P.project(ex.lanes[r.lane][1], r.X, config=cfg)
         torch.cuda.synchronize()
         assert torch.equal(alone, r.Y), f"request {r.rid} (n={r.n}) differs from its alone result"
         err = W.rel_fro_error(r.Y, Pm)
