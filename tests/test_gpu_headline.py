@@ -185,19 +185,69 @@ def test_config2_full_size(handle, method, bar):
         assert err >= 1e-3  # the reference FP16 chain misses 1e-3 on this spectrum
 
 
-@pytest.mark.parametrize("method", ["mixed", "mixed_mx"])
-def test_config3_full_size(handle, method):
+CFG3_SCHEDULES = {
+    "mixed": (dict(method="mixed"), 1e-3),
+    "mixed_mx": (dict(method="mixed_mx"), 1e-3),
+    # the precision study's BF16x3 split products on the FP32 table (~6e-5)
+    "bf16x3": (dict(method="custom", table="fp32", step_format=["bf16x3"] * 10, final_format="bf16x3"), 1e-4),
+}
+
+
+@pytest.mark.parametrize("name", list(CFG3_SCHEDULES))
+def test_config3_full_size(handle, name):
     n = 16384
-    print(f"config 3 {method} n={n}", flush=True)
+    kw, bar = CFG3_SCHEDULES[name]
+    print(f"config 3 {name} n={n}", flush=True)
     X, Pm, _ = W.spectral_matrix(n, 3, "cfg3", device=DEV)
     Xd = X.to(torch.float32)
     del X
     handle.set_seed(3)
-    Y = P.project(handle, Xd, method=method)
+    Y = P.project(handle, Xd, **kw)
     del Xd
     err = W.rel_fro_error(Y, Pm)
     print(f"  rel err {err:.3e}", flush=True)
-    assert err <= 1e-3
+    assert err <= bar
+
+
+def test_config3_e4m3_schedule_diverges_and_says_so(handle):
+    # The precision study's FP8 schedule (e4m3 operands with a per-tensor
+    # scale on the first half of the FP16-table steps, bfloat16 after) keeps
+    # 3 mantissa bits: the first steps' coefficients (8.3, -23.1, 16.5)
+    # amplify the ~6% operand error until iterates leave [-1, 1] and the
+    # quintic diverges (the oracle's emulation of the schedule diverges too).
+    # A synchronous call reports the non-finite result instead of returning it.
+    n = 2048
+    X, _, _ = W.spectral_matrix(n, 3, "cfg3", device=DEV)
+    Xd = X.to(torch.float32)
+    del X
+    cfg = P.default_config("custom", table="fp16", step_format=["e4m3"] * 3 + ["bf16"] * 4, final_format="bf16",
+                           synchronize=1)
+    handle.set_seed(3)
+    with pytest.raises(P.PsdpError) as ei:
+        P.project(handle, Xd, config=cfg)
+    assert ei.value.status == P.ERR_NONFINITE
+
+
+def test_config4_full_batch(handle):
+    # The benchmark's block-diagonal cone: 256 blocks, n log-uniform in
+    # [32, 1024] (seed 4), lambda ~ U[-1, 1]; every block within 1e-3 of its
+    # exact projection, for both shipped schedules.
+    rng = np.random.default_rng(4)
+    sizes = [int(round(float(np.exp(rng.uniform(np.log(32), np.log(1024)))))) for _ in range(256)]
+    xs, ps = [], []
+    for i, n in enumerate(sizes):
+        X64, Pm, _ = W.spectral_matrix(n, 4000 + i, "uniform", device=DEV)
+        xs.append(X64.float().contiguous())
+        ps.append(Pm)
+    torch.cuda.synchronize()
+    for method in ("mixed_mx", "mixed"):
+        print(f"config 4 batch of {len(sizes)} blocks, {method}", flush=True)
+        outs = P.project_batched(handle, xs, method=method)
+        errs = [W.rel_fro_error(y, pm) for y, pm in zip(outs, ps)]
+        print(f"  max rel err {max(errs):.3e}", flush=True)
+        assert max(errs) <= 1e-3
+        for k in (0, 255):
+            assert torch.equal(outs[k], outs[k].T)
 
 
 # ------------------------------------------------------------------ Lanczos bound of the projections
