@@ -536,8 +536,10 @@ def main():
                 "kernel": f"symmetric product, {fmt} operands ("
                           f"{KERNEL_LABEL.get(fmt, 'sym_prod_dma_kernel, 4 waves')}"
                           f" on 256x256 tiles; autotuned tiles "
-                          f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds + the last partial "
-                          "round as 64x64 tiles in a second launch)",
+                          f"{sorted(set(tuned.values())) or 'built-in'}; 256x256 tiles run in whole rounds; a last round "
+                          "that fills at most half the CUs runs as K chunks in a second launch whose partial tiles "
+                          "a reduction adds before the epilogue (split products); avg_ms spans all launches of a "
+                          "product)",
                 "products": cnt, "avg_ms": round(avg_ms, 4), "flops_per_launch": flops,
                 "flops_note": f"{PRODUCTS[fmt]} MFMA product(s) x n^2 (n+1) (upper triangle incl. diagonal)" +
                               (" in binary16-MFMA equivalents: hi.hi on the binary16 MFMA + 2 e4m3 cross "
