@@ -1632,24 +1632,27 @@ __global__ __launch_bounds__(512) void sym_prod_dma8_mx_kernel(ProdDesc d, const
   sym_tile_dma8_mx(d, t, smem);
 }
 
-// Second half of a split product: four workgroups per split 256 x 256 tile
-// ({slot, bi, bj, -}), one per 64-row quarter, add the tile's d.ksplit partial
-// tiles in chunk order (((p0 + p1) + p2) + ...), scale the sum by cscale (the
-// 2^-16 of FP16x3 / F16MX products, inside the epilogue) and run the product's
-// epilogue (mode d.mode) on the quarter.  A thread sums 4 quads at a time and
-// keeps 4 chunks of loads in flight (clamped addresses past the last chunk,
-// whose values are not added), so the reads do not wait one by one.
+// Second half of a split product: SPLIT_REDUCE_WGS workgroups per split
+// 256 x 256 tile ({slot, bi, bj, -}), one per SPLIT_REDUCE_ROWS rows, add the
+// tile's d.ksplit partial tiles in chunk order (((p0 + p1) + p2) + ...), scale
+// the sum by cscale (the 2^-16 of FP16x3 / F16MX products, inside the
+// epilogue) and run the product's epilogue (mode d.mode) on their rows.  A
+// thread sums 4 quads at a time and keeps 4 chunks of loads in flight
+// (clamped addresses past the last chunk, whose values are not added), so the
+// reads do not wait one by one.  32 rows per workgroup keep every F16MX block
+// (32 values of a row, also of the mirrored rows) inside one workgroup.
+constexpr int SPLIT_REDUCE_ROWS = 32, SPLIT_REDUCE_WGS = 256 / SPLIT_REDUCE_ROWS;
 __global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const TileRef* __restrict__ tiles,
                                                                float cscale) {
-  __shared__ __attribute__((aligned(16))) float Ct[64 * 257];
-  const TileRef t = tiles[blockIdx.x >> 2];
-  const int qtr = blockIdx.x & 3;
+  __shared__ __attribute__((aligned(16))) float Ct[SPLIT_REDUCE_ROWS * 257];
+  const TileRef t = tiles[blockIdx.x / SPLIT_REDUCE_WGS];
+  const int part = blockIdx.x % SPLIT_REDUCE_WGS;
   if (t.bi < 0) return;
   const int S = d.ksplit;
-  auto src = gbl((const float*)d.part) + (int64_t)t.mat * S * 65536 + qtr * 64 * 256;
+  auto src = gbl((const float*)d.part) + (int64_t)t.mat * S * 65536 + part * SPLIT_REDUCE_ROWS * 256;
   constexpr int G = 4, KB = 4;
   typedef __attribute__((address_space(1))) const f32x4 gf4;
-  for (int q0 = threadIdx.x; q0 < 64 * 64; q0 += 256 * G) {
+  for (int q0 = threadIdx.x; q0 < SPLIT_REDUCE_ROWS * 64; q0 += 256 * G) {
     f32x4 acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -1691,7 +1694,7 @@ __global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const
     }
   }
   __syncthreads();
-  epilogue_rows<256, 256>(d, t.bi, t.bj, Ct, qtr * 64, 64, cscale);
+  epilogue_rows<256, 256>(d, t.bi, t.bj, Ct, part * SPLIT_REDUCE_ROWS, SPLIT_REDUCE_ROWS, cscale);
 }
 
 template <int SUB>
@@ -1898,7 +1901,7 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
 
 hipError_t launch_split_reduce(int fmt, const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
   const float cscale = (fmt == FMT_F16X3 || fmt == FMT_E4M3 || fmt == FMT_F16MX) ? F16X3_PROD_SCALE : 1.0f;
-  hipLaunchKernelGGL(sym_split_reduce_kernel, dim3(4 * ntiles), dim3(256), 0, s, d, tiles, cscale);
+  hipLaunchKernelGGL(sym_split_reduce_kernel, dim3(SPLIT_REDUCE_WGS * ntiles), dim3(256), 0, s, d, tiles, cscale);
   return hipGetLastError();
 }
 
