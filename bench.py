@@ -199,7 +199,8 @@ def bench_batched(args, P, W, torch):
         xs.append(X64.float().contiguous())
         ps.append(Pm)
     h = P.Handle(0, seed=4)
-    cfg = P.default_config(args.method, synchronize=0)
+    method = args.batched_method
+    cfg = P.default_config(method, synchronize=0)
     outs = P.project_batched(h, xs, config=cfg)
     if args.autotune:
         h.autotune_batched(xs, outs, config=cfg)  # tile classes (not timed)
@@ -218,9 +219,9 @@ def bench_batched(args, P, W, torch):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     errs = [W.rel_fro_error(y, pm) for y, pm in zip(outs, ps)]
-    flops, t_mfma = chain_mfma_ms(args.method, sizes) if args.method in SCHEDULE_FMTS else (None, None)
+    flops, t_mfma = chain_mfma_ms(method, sizes) if method in SCHEDULE_FMTS else (None, None)
     res = {"workload": f"{len(sizes)} blocks, n log-uniform [32,1024] seed 4 (sum n = {sum(sizes)}), "
-                       f"lambda U[-1,1], {args.method}; one grouped launch per product, replayed from a HIP graph",
+                       f"lambda U[-1,1], {method}; one grouped launch per product, replayed from a HIP graph",
            "ms_per_batch": round(ms, 3), "proj_per_s": round(len(sizes) / (ms * 1e-3), 1),
            "max_rel_err": max(errs), "median_rel_err": float(np.median(errs))}
     if flops:
@@ -235,7 +236,7 @@ def bench_batched(args, P, W, torch):
         lanczos_bytes = 41.0 * sum(symv_bytes((n + 63) // 64 * 64) for n in sizes)
         per_step = {"f16x3": 4 + (4 + 4) + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
         per_step["f16mx"] = per_step["f16x3"]  # binary16 hi + e4m3 hi and lo bytes (+ 1/16 B of scales)
-        steps = SCHEDULE_FMTS[args.method]
+        steps = SCHEDULE_FMTS[method]
         bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + lanczos_bytes
         t_hbm = bytes_batch / 8.0e12 * 1e3
         res["roofline"] = {"bound": "hbm" if t_hbm > t_mfma else "mfma", "model_ms_hbm": round(t_hbm, 3),
@@ -391,6 +392,7 @@ def bench_served(args, P, W, torch):
         return [Request(i, int(n), pool[int(n)][i % 4][0], 0.0) for i, n in enumerate(ns)]
 
     ex = TorchExecutor(0, args.method, large_lanes=args.serve_lanes, seed=5, session_bytes=8 << 30,
+                       session_method=args.batched_method,
                        ticks_ahead=args.serve_ticks_ahead)
     ex.warm(sizes)
     budget = 64 << 30
@@ -418,7 +420,8 @@ def bench_served(args, P, W, torch):
     st = latency_stats(done)
     e = torch.stack(errs).cpu().numpy() if errs else np.zeros(1)
     res = {"workload": f"{len(reqs)} requests, Poisson at {rate:.1f}/s = {args.serve_load} x "
-                       f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); {args.method}; "
+                       f"burst capacity {cap:.1f}/s; n in {sizes} w.p. {weights}; lambda N(0,1); "
+                       f"{args.batched_method} for n <= 1024, {args.method} for larger; "
                        f"n <= 1024 in one continuous-batching session (requests join between products), "
                        f"n > 1024 on {args.serve_lanes} lane(s)",
            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
@@ -436,6 +439,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--method", default="mixed_mx")
+    ap.add_argument("--batched-method", default="mixed",
+                    help="schedule of the config-4 batch and of the served loop's small requests: mixed "
+                         "(binary16x3 steps 0-3) runs the 64x64 tiles of n <= 1024 blocks faster than mixed_mx, "
+                         "whose F16MX products pay off on 256x256 tiles")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="CPU baseline block size (the whole matrix by default; 0: skip)")

@@ -68,19 +68,22 @@ class TorchExecutor:
     # p99 285 ms, burst capacity 643/s); with one lane 9.5 / 45 ms at 594/s.
     # Without large requests in the stream the session alone gives 1.1 / 3.3 ms.
     def __init__(self, device=0, method="mixed", large_lanes=1, seed=5, session_bytes=6 << 30,
-                 small_max=1024, ticks_ahead=4):
+                 small_max=1024, ticks_ahead=4, session_method=None):
         import torch
         from . import Handle, Session, default_config, workspace_size
         self.torch = torch
         self.dev = torch.device("cuda", device)
-        self.cfg = default_config(method, synchronize=0)
+        self.cfg = default_config(method, synchronize=0)  # the lanes' schedule
         self.method = method
+        # the session's schedule (small requests), the lanes' unless given
+        self.session_method = session_method or method
+        self.sess_cfg = default_config(self.session_method, synchronize=0)
         self.small_max = small_max
         self.ticks_ahead = ticks_ahead
         self._ws = workspace_size
         self.sess_stream = torch.cuda.Stream(device=self.dev)
         self.sess_handle = Handle(device, stream=self.sess_stream, seed=seed)
-        self.session = Session(self.sess_handle, session_bytes, config=self.cfg)
+        self.session = Session(self.sess_handle, session_bytes, config=self.sess_cfg)
         self.session_bytes = int(session_bytes)
         self.tickets: Dict[int, Request] = {}
         self.lanes = []
