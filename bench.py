@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the PSD-projection hot path on one MI355X.
 
-Workload (config 3 of the benchmark set): one symmetric n=16384 matrix
+Workload: one symmetric n=16384 matrix
 X = Q diag(lam) Q^T stored in float32, lam ~ U[-5,5] with 10% of the
 eigenvalues in [-1e-3, 1e-3], Haar Q.  A step is one full projection
 (Lanczos bound + composite polynomial chain + output) with the shipped
@@ -96,7 +96,7 @@ def cpu_full_baseline(n):
     import torch
     from oracle import psdp_oracle as O
     from psd_projection_amd import workloads as W
-    X64, Pm, _ = W.spectral_matrix(n, 3, "cfg3")
+    X64, Pm, _ = W.spectral_matrix(n, 3, "uniform_cluster")
     A = X64.cpu().numpy()
     del X64
     Y, _, stages = O.eig_FP64_psd_staged(A, log=log)
@@ -122,7 +122,7 @@ def measured_traffic(n, fmt):
 
 
 def precision_study(args, P, W, torch, h, X, Y, Pm, out):
-    """Config 3: the FP32-table chain with FP32 MFMA, BF16x3 and BF16x9 split
+    """Precision study at n=16384 (uniform_cluster spectrum): the FP32-table chain with FP32 MFMA, BF16x3 and BF16x9 split
     products, and e4m3 operands (per-tensor scale 2^8) on the first half of the
     steps with bfloat16 on the rest; next to the shipped schedule.  e4m3
     operand values are computed on the binary16 MFMA (products of e4m3 values
@@ -188,7 +188,7 @@ def chain_mfma_ms(method, sizes):
 
 
 def bench_batched(args, P, W, torch):
-    """Config 4: 256 blocks, n log-uniform in [32, 1024] (seed 4), lambda ~ U[-1,1],
+    """The block-diagonal batch: 256 blocks, n log-uniform in [32, 1024] (seed 4), lambda ~ U[-1,1],
     one grouped launch per filter-chain product for all blocks."""
     import numpy as np
     rng = np.random.default_rng(4)
@@ -320,8 +320,8 @@ def _timed(P, torch, h, X, Y, cfg, reps):
 
 
 def bench_small_configs(args, P, W, torch):
-    """Config 1 (n=512, lambda ~ U[-1,1], plain composite_FP32, also the EVD
-    projection on the host cores) and config 2 (n=8192, lambda ~ N(0,1) scaled
+    """n=512 (lambda ~ U[-1,1], plain composite_FP32, also the EVD
+    projection on the host cores) and n=8192 (lambda ~ N(0,1) scaled
     to max|lambda| = 10 with 30% in [-1e-2, 1e-2]: plain FP32 chain against the
     reference's FP16 chain and the shipped mixed schedule).  Errors are
     relative Frobenius errors against the exact projection Q max(L,0) Q^T."""
@@ -341,11 +341,11 @@ def bench_small_configs(args, P, W, torch):
         t0 = time.perf_counter()
         O.eig_FP64_psd(host)
         cpu.append((time.perf_counter() - t0) * 1e3)
-    out["cfg1_n512"] = {"workload": "n=512, Haar Q (seed 1), lambda U[-1,1], float32, composite_FP32 auto-scale",
+    out["n512_composite_fp32"] = {"workload": "n=512, Haar Q (seed 1), lambda U[-1,1], float32, composite_FP32 auto-scale",
                         "ms": round(ms, 3), "rel_err": err, "cpu_eig_fp64_ms": round(sorted(cpu)[1], 2)}
-    log(f"config 1: {out['cfg1_n512']}")
+    log(f"n=512: {out['n512_composite_fp32']}")
     del X, Y, X64, Pm
-    X64, Pm, _ = W.spectral_matrix(8192, 2, "cfg2")
+    X64, Pm, _ = W.spectral_matrix(8192, 2, "normal_cluster")
     X = X64.float().contiguous()
     del X64
     Y = torch.empty_like(X)
@@ -361,8 +361,8 @@ def bench_small_configs(args, P, W, torch):
     res["fp16_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["composite_FP16"]["ms"], 2)
     res["mixed_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["mixed"]["ms"], 2)
     res["mixed_mx_speedup_vs_fp32"] = round(res["composite_FP32"]["ms"] / res["mixed_mx"]["ms"], 2)
-    out["cfg2_n8192"] = res
-    log(f"config 2: {res}")
+    out["n8192_normal_cluster"] = res
+    log(f"n=8192: {res}")
     del X, Y, Pm
     h.close()
     torch.cuda.empty_cache()
@@ -370,7 +370,7 @@ def bench_small_configs(args, P, W, torch):
 
 
 def bench_served(args, P, W, torch):
-    """Config 5: Poisson stream of requests, n in {64,256,1024,4096,8192} with
+    """Served stream: Poisson arrivals of requests, n in {64,256,1024,4096,8192} with
     weights {.4,.3,.15,.1,.05}, lambda ~ N(0,1), FP32 in/out; admission by HBM
     budget, continuous batching over stream lanes.  Inputs come from a pool of
     seeded matrices per size (request i uses pool entry i mod 4)."""
@@ -440,17 +440,17 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--method", default="mixed_mx")
     ap.add_argument("--batched-method", default="mixed",
-                    help="schedule of the config-4 batch and of the served loop's small requests: mixed "
+                    help="schedule of the 256-block batch and of the served loop's small requests: mixed "
                          "(binary16x3 steps 0-3) runs the 64x64 tiles of n <= 1024 blocks faster than mixed_mx, "
                          "whose F16MX products pay off on 256x256 tiles")
     ap.add_argument("--fp32-steps", type=int, default=1, help="timed runs of the FP32 chain (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="CPU baseline block size (the whole matrix by default; 0: skip)")
     ap.add_argument("--precision-study", type=int, default=1,
-                    help="config 3 variants (BF16x3, BF16x9, e4m3 first half + bf16) at the same size (0: skip)")
-    ap.add_argument("--batched", type=int, default=256, help="config 4 block count (0: skip)")
+                    help="precision-study variants (BF16x3, BF16x9, e4m3 first half + bf16) at the same size (0: skip)")
+    ap.add_argument("--batched", type=int, default=256, help="block count of the block-diagonal batch (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=10000,
-                    help="config 5 requests in the timed Poisson stream (0: skip; the config is 10000)")
+                    help="requests in the timed Poisson stream (0: skip; the workload is 10000)")
     ap.add_argument("--small-configs", type=int, default=1,
                     help="configs 1 (n=512) and 2 (n=8192, FP32 vs FP16 chain) (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
@@ -484,7 +484,7 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    X64, Pm, lam = W.spectral_matrix(n, 3, "cfg3", basis=args.basis)
+    X64, Pm, lam = W.spectral_matrix(n, 3, "uniform_cluster", basis=args.basis)
     X = X64.float().contiguous()
     torch.cuda.synchronize()
     log(f"workload n={n} generated in {time.perf_counter() - t0:.1f} s")
@@ -569,7 +569,7 @@ def main():
            "data": ("synthetic: Haar Q (QR of a seeded Gaussian, torch FP64 on the GPU)" if args.basis == "haar"
                     else "synthetic: Householder Q (one seeded reflector)") +
                    ", lambda ~ U[-5,5] with 10% in [-1e-3,1e-3]; float32 storage",
-           "config": {"workload": f"single n={n} projection (config 3 spectrum), {args.method} schedule, "
+           "config": {"workload": f"single n={n} projection (uniform_cluster spectrum: U[-5,5], 10% in [-1e-3,1e-3]), {args.method} schedule, "
                                   "Lanczos 20 it + 7-step FP16-table chain + final product",
                       "n": n, "method": args.method},
            "rel_err": err, "lanczos": {"lo": lo, "up": up, "iterations": iters},
@@ -612,7 +612,7 @@ def main():
     del X, Y, Pm
     torch.cuda.empty_cache()
     if args.small_configs:
-        out["configs_1_2"] = bench_small_configs(args, P, W, torch)
+        out["n512_n8192"] = bench_small_configs(args, P, W, torch)
     if args.batched > 0:
         out["batched"] = bench_batched(args, P, W, torch)
     if args.serve_requests > 0:

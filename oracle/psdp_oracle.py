@@ -133,7 +133,7 @@ def round_e4m3(x):
     """float32 -> nearest FP8 E4M3FN value (OCP: 3 mantissa bits, exponent
     bias 7, normal exponents >= -6, subnormal quantum 2**-9, max 448, values
     beyond saturate), ties to even -> float32.  Not a reference format (the
-    reference has no FP8 path); the config-3 precision study asks for it."""
+    reference has no FP8 path); the FP8 precision study asks for it."""
     x = np.asarray(x, dtype=np.float32)
     a = np.abs(x).astype(np.float64)
     out = np.empty_like(x)
@@ -808,8 +808,8 @@ def haar_orthogonal(n, seed):
     return Q * np.sign(np.diag(R))
 
 
-def config1_input(n=512):
-    """Config 1: n=512, Q from QR of N(0,1) seed 1, lambda ~ U[-1,1] seed 1001."""
+def uniform512_input(n=512):
+    """The n=512 case: Q from QR of N(0,1) seed 1, lambda ~ U[-1,1] seed 1001."""
     Q = haar_orthogonal(n, 1)
     lam = np.random.default_rng(1001).uniform(-1.0, 1.0, n)
     X = (Q * lam) @ Q.T
@@ -818,14 +818,14 @@ def config1_input(n=512):
 
 
 def clustered_spectrum(n, seed, kind):
-    """Spectra of configs 2/3 (scaled by the same rule at any n)."""
+    """The clustered and plain spectra of the workloads (scaled by the same rule at any n)."""
     rng = np.random.default_rng(seed + 100)
-    if kind == "cfg2":
+    if kind == "normal_cluster":
         lam = rng.standard_normal(n)
         lam *= 10.0 / np.max(np.abs(lam))
         idx = rng.choice(n, int(round(0.3 * n)), replace=False)
         lam[idx] = rng.uniform(-1e-2, 1e-2, idx.size)
-    elif kind == "cfg3":
+    elif kind == "uniform_cluster":
         lam = rng.uniform(-5.0, 5.0, n)
         idx = rng.choice(n, int(round(0.1 * n)), replace=False)
         lam[idx] = rng.uniform(-1e-3, 1e-3, idx.size)

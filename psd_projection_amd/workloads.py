@@ -1,4 +1,4 @@
-"""Synthetic workloads of the benchmark configurations, generated on the GPU.
+"""Synthetic benchmark workloads, generated on the GPU.
 
 Every matrix is X = Q diag(lam) Q^T with Q Haar-distributed (QR of a seeded
 Gaussian, sign-fixed so that diag(R) > 0), symmetrised; the exact projection
@@ -11,10 +11,11 @@ import numpy as np
 
 
 def spectrum(n: int, seed: int, kind: str) -> np.ndarray:
-    """Eigenvalues of the configurations:
-    uniform: U[-1,1] (configs 1, 4); cfg2: N(0,1) scaled to max|lam| = 10 with
-    30% replaced by U[-1e-2,1e-2]; cfg3: U[-5,5] with 10% replaced by
-    U[-1e-3,1e-3]; normal: N(0,1) (config 5); even10: evenly spaced in
+    """Eigenvalues of the benchmark workloads:
+    uniform: U[-1,1] (the n=512 case and the 256-block batch);
+    normal_cluster: N(0,1) scaled to max|lam| = 10 with 30% replaced by
+    U[-1e-2,1e-2]; uniform_cluster: U[-5,5] with 10% replaced by
+    U[-1e-3,1e-3]; normal: N(0,1) (the served request stream); even10: evenly spaced in
     [-10, 10], the spectrum of the reference's test generator
     (test/test_utils.hpp:20-164, radius 10)."""
     if kind == "even10":
@@ -24,13 +25,13 @@ def spectrum(n: int, seed: int, kind: str) -> np.ndarray:
         return rng.uniform(-1.0, 1.0, n)
     if kind == "normal":
         return rng.standard_normal(n)
-    if kind == "cfg2":
+    if kind == "normal_cluster":
         lam = rng.standard_normal(n)
         lam *= 10.0 / np.max(np.abs(lam))
         idx = rng.choice(n, int(round(0.3 * n)), replace=False)
         lam[idx] = rng.uniform(-1e-2, 1e-2, idx.size)
         return lam
-    if kind == "cfg3":
+    if kind == "uniform_cluster":
         lam = rng.uniform(-5.0, 5.0, n)
         idx = rng.choice(n, int(round(0.1 * n)), replace=False)
         lam[idx] = rng.uniform(-1e-3, 1e-3, idx.size)

@@ -118,7 +118,7 @@ def test_eigenpairs_kat_5x5(handle, row):
 
 
 @pytest.mark.parametrize("row", ["first", "last"])
-@pytest.mark.parametrize("n,max_iter,kind", [(300, 25, "normal"), (700, 40, "cfg2")])
+@pytest.mark.parametrize("n,max_iter,kind", [(300, 25, "normal"), (700, 40, "normal_cluster")])
 def test_eigenpairs_short_run_matches_oracle(handle, row, n, max_iter, kind):
     X, _, lam = O.spectral_input(n, 21, kind)
     seed = 41
@@ -185,7 +185,7 @@ def test_eigenpairs_picks_by_absolute_value(handle):
 
 
 def test_eigenpairs_upper_bound_only(handle):
-    X, _, lam = O.spectral_input(256, 25, "cfg3")
+    X, _, lam = O.spectral_input(256, 25, "uniform_cluster")
     seed = 43
     handle.set_seed(seed)
     _, _, b = P.compute_eigenpairs(handle, to_dev(X), 0, upper_bound_only=True, max_iter=30, ortho_tol=1e-5)

@@ -13,7 +13,8 @@ their result must be
     test_gpu_parity.py), or, at sizes the emulation cannot reach, within the
     method's bar against the exact projection.
 
-Then the full sizes of the benchmark configurations against the closed form
+Then the benchmark workloads at full size (n = 8192, n = 16384, the 256-block
+batch) against the closed form
 Q max(L, 0) Q^T (generated on the device in float64), and the float32 Lanczos
 bound that the projections use against the oracle's bound on the same
 float32-rounded matrix and start vector (relative 1e-9).
@@ -78,7 +79,7 @@ def test_big_tiles_projection_vs_oracle(handle, name):
     # n = 2304: 9 x 9 tiles of 256 (45 upper-triangle tiles, no tail launch)
     n, seed = 2304, 31
     print(f"big-tile projection {name} n={n}", flush=True)
-    X, Q, lam = O.spectral_input(n, 3, "cfg3")
+    X, Q, lam = O.spectral_input(n, 3, "uniform_cluster")
     X32 = X.astype(np.float32)
     Xd = torch.from_numpy(X32).to(DEV)
     Y256 = project_tiled(handle, Xd, name, n, 256, seed)
@@ -109,7 +110,7 @@ def test_big_tiles_projection_bitwise(handle, n, name, bar):
     # same bar.
     seed = 8
     print(f"big-tile projection {name} n={n}", flush=True)
-    X, Pm, _ = W.spectral_matrix(n, 40 + n % 7, "cfg3", device=DEV)
+    X, Pm, _ = W.spectral_matrix(n, 40 + n % 7, "uniform_cluster", device=DEV)
     Xd = X.to(torch.float32)
     del X
     handle.set_split_k(False)
@@ -166,14 +167,14 @@ def test_split_products_match_whole_k(handle, n, fmt):
 # ------------------------------------------------------------------ full sizes
 @pytest.mark.parametrize("method,bar", [("composite_FP32", 1e-4), ("composite_FP16", 5e-3), ("mixed", 1e-3),
                                         ("mixed_mx", 1e-3)])
-def test_config2_full_size(handle, method, bar):
+def test_normal_cluster_n8192_full_size(handle, method, bar):
     # n = 8192, lambda ~ N(0,1) scaled to max 10 with 30% in [-1e-2, 1e-2].
     # The reference's FP16 chain is at 2-3e-3 on this spectrum (above 1e-3,
     # as its op order predicts); the FP32 chain and the shipped schedule stay
     # within their bars.
     n = 8192
-    print(f"config 2 {method} n={n}", flush=True)
-    X, Pm, _ = W.spectral_matrix(n, 2, "cfg2", device=DEV)
+    print(f"normal_cluster {method} n={n}", flush=True)
+    X, Pm, _ = W.spectral_matrix(n, 2, "normal_cluster", device=DEV)
     Xd = X.to(torch.float32)
     del X
     handle.set_seed(2)
@@ -185,7 +186,7 @@ def test_config2_full_size(handle, method, bar):
         assert err >= 1e-3  # the reference FP16 chain misses 1e-3 on this spectrum
 
 
-CFG3_SCHEDULES = {
+N16384_SCHEDULES = {
     "mixed": (dict(method="mixed"), 1e-3),
     "mixed_mx": (dict(method="mixed_mx"), 1e-3),
     # the precision study's BF16x3 split products on the FP32 table (~6e-5)
@@ -193,12 +194,12 @@ CFG3_SCHEDULES = {
 }
 
 
-@pytest.mark.parametrize("name", list(CFG3_SCHEDULES))
-def test_config3_full_size(handle, name):
+@pytest.mark.parametrize("name", list(N16384_SCHEDULES))
+def test_uniform_cluster_n16384_full_size(handle, name):
     n = 16384
-    kw, bar = CFG3_SCHEDULES[name]
-    print(f"config 3 {name} n={n}", flush=True)
-    X, Pm, _ = W.spectral_matrix(n, 3, "cfg3", device=DEV)
+    kw, bar = N16384_SCHEDULES[name]
+    print(f"uniform_cluster {name} n={n}", flush=True)
+    X, Pm, _ = W.spectral_matrix(n, 3, "uniform_cluster", device=DEV)
     Xd = X.to(torch.float32)
     del X
     handle.set_seed(3)
@@ -209,7 +210,7 @@ def test_config3_full_size(handle, name):
     assert err <= bar
 
 
-def test_config3_e4m3_schedule_diverges_and_says_so(handle):
+def test_e4m3_schedule_diverges_and_says_so(handle):
     # The precision study's FP8 schedule (e4m3 operands with a per-tensor
     # scale on the first half of the FP16-table steps, bfloat16 after) keeps
     # 3 mantissa bits: the first steps' coefficients (8.3, -23.1, 16.5)
@@ -217,7 +218,7 @@ def test_config3_e4m3_schedule_diverges_and_says_so(handle):
     # quintic diverges (the oracle's emulation of the schedule diverges too).
     # A synchronous call reports the non-finite result instead of returning it.
     n = 2048
-    X, _, _ = W.spectral_matrix(n, 3, "cfg3", device=DEV)
+    X, _, _ = W.spectral_matrix(n, 3, "uniform_cluster", device=DEV)
     Xd = X.to(torch.float32)
     del X
     cfg = P.default_config("custom", table="fp16", step_format=["e4m3"] * 3 + ["bf16"] * 4, final_format="bf16",
@@ -228,7 +229,7 @@ def test_config3_e4m3_schedule_diverges_and_says_so(handle):
     assert ei.value.status == P.ERR_NONFINITE
 
 
-def test_config4_full_batch(handle):
+def test_256_block_batch_full(handle):
     # The benchmark's block-diagonal cone: 256 blocks, n log-uniform in
     # [32, 1024] (seed 4), lambda ~ U[-1, 1]; every block within 1e-3 of its
     # exact projection, for both shipped schedules.
@@ -241,7 +242,7 @@ def test_config4_full_batch(handle):
         ps.append(Pm)
     torch.cuda.synchronize()
     for method in ("mixed_mx", "mixed"):
-        print(f"config 4 batch of {len(sizes)} blocks, {method}", flush=True)
+        print(f"batch of {len(sizes)} blocks, {method}", flush=True)
         outs = P.project_batched(handle, xs, method=method)
         errs = [W.rel_fro_error(y, pm) for y, pm in zip(outs, ps)]
         print(f"  max rel err {max(errs):.3e}", flush=True)
@@ -252,7 +253,7 @@ def test_config4_full_batch(handle):
 
 # ------------------------------------------------------------------ Lanczos bound of the projections
 @pytest.mark.parametrize("mode", ["reference", "tight"])
-@pytest.mark.parametrize("n,kind", [(300, "normal"), (2100, "cfg3"), (4100, "cfg2")])
+@pytest.mark.parametrize("n,kind", [(300, "normal"), (2100, "uniform_cluster"), (4100, "normal_cluster")])
 def test_projection_bound_matches_oracle(handle, n, kind, mode):
     # The projections bound the float32-rounded matrix with the half-read
     # float32 GEMV (float64 accumulation); the oracle runs the same Lanczos in
@@ -293,12 +294,12 @@ def test_batched_bounds_match_oracle(handle, mode):
         handle.last_bounds_batched(len(sizes) + 1)
 
 
-def test_tight_bound_projection_cfg3(handle):
-    # Opt-in tight bound: on the clustered cfg-3 spectrum (||X||_2 = 5) the
+def test_tight_bound_projection_uniform_cluster(handle):
+    # Opt-in tight bound: on the uniform_cluster spectrum (||X||_2 = 5) the
     # reference formula gives up ~ 7; the tight one stays within the Ritz
     # error of 5 and the projection keeps its accuracy.
     n = 2100
-    X, Q, lam = O.spectral_input(n, 3, "cfg3")
+    X, Q, lam = O.spectral_input(n, 3, "uniform_cluster")
     Xd = torch.from_numpy(X.astype(np.float32)).to(DEV)
     Pt = O.closed_form_projection(Q, lam)
     handle.set_seed(4)
@@ -310,7 +311,7 @@ def test_tight_bound_projection_cfg3(handle):
     nrm = np.abs(lam).max()
     e_ref = O.rel_fro_error(Y_ref.cpu().numpy(), Pt)
     e_t = O.rel_fro_error(Y_t.cpu().numpy(), Pt)
-    print(f"cfg3 n={n}: up {up_ref:.4f} (reference) {up_t:.4f} (tight), ||X|| {nrm:.4f}; "
+    print(f"uniform_cluster n={n}: up {up_ref:.4f} (reference) {up_t:.4f} (tight), ||X|| {nrm:.4f}; "
           f"err {e_ref:.3e} / {e_t:.3e}", flush=True)
     assert nrm <= up_t < 1.01 * nrm < up_ref
     assert e_t <= 1e-3

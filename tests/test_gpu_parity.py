@@ -132,7 +132,7 @@ def test_lanczos_kat_5x5(handle):
     assert up == pytest.approx(up_o, rel=1e-9)
 
 
-@pytest.mark.parametrize("n,kind,seed", [(300, "normal", 3), (1000, "cfg2", 4), (777, "cfg3", 5)])
+@pytest.mark.parametrize("n,kind,seed", [(300, "normal", 3), (1000, "normal_cluster", 4), (777, "uniform_cluster", 5)])
 def test_lanczos_matches_oracle(handle, n, kind, seed):
     X, Q, lam = O.spectral_input(n, seed, kind)
     handle.set_seed(seed)
@@ -209,16 +209,16 @@ def test_reference_uniform_autoscaled_1024(handle, fn, method):
 @pytest.mark.parametrize("method,kind,n,bar", [
     ("composite_FP32", "uniform", 512, 2e-5),
     ("composite_FP32_emulated", "uniform", 512, 2e-5),
-    ("composite_FP16", "cfg2", 512, None),
+    ("composite_FP16", "normal_cluster", 512, None),
     ("composite_FP16", "uniform", 512, None),
-    ("mixed", "cfg2", 1024, 1e-3),
-    ("mixed", "cfg3", 1024, 1e-3),
+    ("mixed", "normal_cluster", 1024, 1e-3),
+    ("mixed", "uniform_cluster", 1024, 1e-3),
     ("mixed", "uniform", 640, 1e-3),
-    ("mixed", "cfg3", 2100, 1e-3),
+    ("mixed", "uniform_cluster", 2100, 1e-3),
 ])
 def test_projection_accuracy(handle, method, kind, n, bar):
     print(f"projection {method} {kind} n={n}", flush=True)
-    X, Q, lam = O.spectral_input(n, 1 if kind == "uniform" else (2 if kind == "cfg2" else 3), kind)
+    X, Q, lam = O.spectral_input(n, 1 if kind == "uniform" else (2 if kind == "normal_cluster" else 3), kind)
     Ptrue = O.closed_form_projection(Q, lam)
     seed = 5
     handle.set_seed(seed)
@@ -234,13 +234,13 @@ def test_projection_accuracy(handle, method, kind, n, bar):
     assert err >= 0.7 * e_or - 2e-6
 
 
-def test_config1_fp32_io(handle):
-    # config 1: n=512, FP32 storage, composite_FP32 with the Lanczos bound
-    X, Q, lam = O.config1_input(512)
+def test_n512_fp32_io(handle):
+    # n=512, FP32 storage, composite_FP32 with the Lanczos bound
+    X, Q, lam = O.uniform512_input(512)
     Ptrue = O.closed_form_projection(Q, lam)
     Y = run_projection(handle, X, "composite_FP32", dtype=torch.float32)
     err = O.rel_fro_error(Y, Ptrue)
-    print("config1 fp32 rel err", err)
+    print("n512 fp32 rel err", err)
     assert err <= 2e-5
 
 

@@ -168,8 +168,8 @@ def test_fp32_chain_error_level():
 
 
 def test_fp16_chain_error_level_clustered():
-    # The reference FP16 chain misses 1e-3 on a cfg2-like clustered spectrum.
-    X, Q, lam = O.spectral_input(512, 2, "cfg2")
+    # The reference FP16 chain misses 1e-3 on the normal_cluster spectrum.
+    X, Q, lam = O.spectral_input(512, 2, "normal_cluster")
     P = O.closed_form_projection(Q, lam)
     _, up = O.lanczos_two_norm(X)
     err = O.rel_fro_error(O.composite_FP16(X / up) * up, P)
@@ -250,7 +250,7 @@ def test_lanczos_tight_complete_krylov_space():
     assert up_ref == pytest.approx(np.sqrt(nrm ** 2 + abs(nrm - nrm ** 2)), rel=1e-6)
 
 
-@pytest.mark.parametrize("kind", ["uniform", "normal", "cfg3"])
+@pytest.mark.parametrize("kind", ["uniform", "normal", "uniform_cluster"])
 def test_lanczos_tight_brackets_norm(kind):
     X, Q, lam = O.spectral_input(300, 11, kind)
     nrm = np.abs(lam).max()
@@ -310,7 +310,7 @@ def test_average_mode_tracks_reference_symmetrize():
     # Averaging every product with its transpose keeps the reference's error
     # (which symmetrizes A once per step by averaging); mirroring the upper
     # triangle of plain-FP16 products roughly doubles it.
-    X, Q, lam = O.spectral_input(384, 2, "cfg2")
+    X, Q, lam = O.spectral_input(384, 2, "normal_cluster")
     Pt = O.closed_form_projection(Q, lam)
     _, up = O.lanczos_two_norm(X)
     Xs = (X / up).astype(np.float32)
