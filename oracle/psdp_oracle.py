@@ -377,6 +377,11 @@ def lanczos_two_norm(A, max_iter=20, tol=1e-10, q0=None, return_state=False, kee
         w = w - alpha[k] * q - beta_old * v_old
         beta[k] = np.linalg.norm(w)
         if beta[k] <= tol * alpha[k] and k > 1:
+            # with keep_exact, iteration n - 1 of an n x n matrix is kept: the
+            # Krylov space is the whole space and beta is rounding noise
+            # (the reference drops it, which can leave up < ||A|| for n <= 20)
+            if keep_exact and k == n - 1 and alpha[k] > 0.0:
+                nb += 1
             break
         v_old = q.copy()
         if beta[k] == 0.0:
@@ -858,7 +863,9 @@ PINNING = {
     "lanczos_two_norm": "5x5 KAT (7.089012550562305) bracketed; up formula vs direct "
                         "recomputation from T; lo <= ||A|| on random inputs; zero matrix -> (0,1); "
                         "tight mode: a complete Krylov space gives up = ||A||_2 to 1e-12 (the reference "
-                        "residual of the same pair does not), up >= ||A||_2 >= lo on random spectra",
+                        "residual of the same pair does not), up >= ||A||_2 >= lo on random spectra; "
+                        "keep_exact: a 3x3 input whose reference-rule bound undercuts ||A||_2 gets "
+                        "lo = ||A||_2 and up >= ||A||_2, larger inputs are unchanged",
     "eig_FP64_psd": "diag KAT, PSD fixed point, closed form Q max(L,0) Q^T",
     "eig_FP64_psd_staged": "equals eig_FP64_psd (scipy eigh, driver evd) and the closed form on random and "
                            "diagonal inputs; eigenvalues equal numpy eigvalsh",

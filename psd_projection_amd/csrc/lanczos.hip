@@ -5,9 +5,9 @@
 // graph capturable):
 //   * Lanczos on A^T A = A A (A symmetric), no reorthogonalisation
 //     (src/lanczos.cu:116-169), the break rule of :145 and :152-159 evaluated
-//     on the device (except that an exact invariant subspace keeps its
-//     iteration, see step_body); after a break the remaining launches are
-//     no-ops;
+//     on the device (except that an exact invariant subspace, or the last
+//     iteration of an n x n matrix with n <= max_iter, keeps its iteration,
+//     see step_body); after a break the remaining launches are no-ops;
 //   * T = tridiag(alpha, beta) (src/lanczos.cu:16-38), its largest eigenpair
 //     by multisection + inverse iteration in FP64 (replaces cusolverDnDsyevd
 //     at :206-214; T is built from zeros, unlike the unzeroed buffer there);
@@ -316,8 +316,9 @@ __device__ void init_body(LanczosState* st, const LanczosVecs& v, int n, int m, 
   }
 }
 
-// One Lanczos iteration after w = A (A q) (src/lanczos.cu:130-168).
-__device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int m, int max_iter, double tol) {
+// One Lanczos iteration after w = A (A q) (src/lanczos.cu:130-168); n is the
+// logical size of the matrix.
+__device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int n, int m, int max_iter, double tol) {
   __shared__ double red[16];
   if (st->done || k >= max_iter) return;
   double s = 0.0;
@@ -338,8 +339,16 @@ __device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int m, 
   // drops it, so a start vector that is an eigenvector (any 1x1 matrix) is
   // taken for the zero matrix and gets up = 1; here that iteration is kept
   // (its Ritz value is exact) unless alpha == 0 (the zero matrix itself).
-  const bool drop = bad || (beta <= tol * alpha && k > 1) || (beta == 0.0 && !(alpha > 0.0));
-  const bool exact = !drop && beta == 0.0;
+  // Likewise iteration n - 1 of an n x n matrix (n <= max_iter): the Krylov
+  // space is then the whole space, beta is rounding noise, and dropping the
+  // iteration leaves theta from an (n-1)-dimensional space, which can fall
+  // below ||A||^2 by more than the residual term adds back (the bound then
+  // undercuts ||A|| and the filter chain diverges on a 3 x 3 input); kept,
+  // theta is the exact largest eigenvalue of A A.
+  const bool conv = beta <= tol * alpha && k > 1;
+  const bool full = conv && k == n - 1 && alpha > 0.0;
+  const bool drop = bad || (conv && !full) || (beta == 0.0 && !(alpha > 0.0));
+  const bool exact = !drop && (beta == 0.0 || full);
   const bool stop = drop || exact;
   if (!stop) {
     const double inv = 1.0 / beta;
@@ -546,9 +555,9 @@ __global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, La
                                                             uint64_t seed) {
   init_body(st, v, n, m, seed);
 }
-__global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int m,
+__global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int n, int m,
                                                             int max_iter, double tol) {
-  step_body(st, v, k, m, max_iter, tol);
+  step_body(st, v, k, n, m, max_iter, tol);
 }
 __global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) { ritz_body(st); }
 __global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st, LanczosVecs v, int m) {
@@ -566,7 +575,7 @@ __global__ __launch_bounds__(1024) void lanczos_init_batched(const LanczosJob* j
 }
 __global__ __launch_bounds__(1024) void lanczos_step_batched(const LanczosJob* jobs, int k, double tol) {
   const LanczosJob& J = jobs[blockIdx.x];
-  step_body(J.st, J.v, k, J.m, J.max_iter, tol);
+  step_body(J.st, J.v, k, J.n, J.m, J.max_iter, tol);
 }
 __global__ __launch_bounds__(64) void lanczos_ritz_batched(const LanczosJob* jobs) { ritz_body(jobs[blockIdx.x].st); }
 __global__ __launch_bounds__(256) void ritz_vector_batched(const LanczosJob* jobs) {
@@ -624,7 +633,7 @@ hipError_t launch_lanczos(const void* A, int a_f64, int64_t ld, int n, int m, in
       launch_symv((const float*)A, ld, m, v.q, v.w1, v.part, &st->done, s);
       launch_symv((const float*)A, ld, m, v.w1, v.w, v.part, &st->done, s);
     }
-    hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, s, st, v, k, m, max_iter, tol);
+    hipLaunchKernelGGL(lanczos_step_kernel, dim3(1), dim3(1024), 0, s, st, v, k, n, m, max_iter, tol);
   }
   hipLaunchKernelGGL(lanczos_ritz_kernel, dim3(1), dim3(64), 0, s, st);
   hipLaunchKernelGGL(ritz_vector_kernel, dim3((m + 255) / 256), dim3(256), 0, s, st, v, m);

@@ -267,6 +267,35 @@ def test_two_by_two(handle):
     assert np.abs(Y - expect).max() < 1e-4
 
 
+def three_by_three_undercut():
+    # A 3 x 3 input whose bound under the reference's convergence rule
+    # (src/lanczos.cu:145 drops the third, Krylov-space-completing iteration)
+    # undercuts ||X||_2 with the start vector of seed 9 (see test_oracle.py)
+    rng = np.random.default_rng(148)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    lam = rng.uniform(-5.0, 5.0, 3)
+    X = (Q * lam) @ Q.T
+    return 0.5 * (X + X.T), Q, lam
+
+
+@pytest.mark.parametrize("method", ["mixed_mx", "composite_FP32"])
+def test_three_by_three_bound_covers_norm(handle, method):
+    X, Q, lam = three_by_three_undercut()
+    handle.set_seed(9)
+    print(f"3x3 bound {method}", flush=True)
+    Xd = torch.tensor(X, dtype=torch.float32, device=DEV)
+    Y = P.project(handle, Xd, method=method)
+    lo, up, it = handle.last_bound()
+    Xf = X.astype(np.float32).astype(np.float64)
+    lo_o, up_o = O.lanczos_two_norm(Xf, 20, 1e-10, q0=start_vector(9, 3))
+    nrm = np.linalg.norm(Xf, 2)
+    assert it == 3 and up == pytest.approx(up_o, rel=1e-9) and up >= nrm
+    assert lo == pytest.approx(nrm, rel=1e-9)
+    Pm = (Q * np.maximum(lam, 0.0)) @ Q.T
+    err = np.linalg.norm(Y.double().cpu().numpy() - Pm) / np.linalg.norm(Pm)
+    assert err <= (1e-3 if method == "mixed_mx" else 1e-4), err
+
+
 def test_nonfinite_input_reported(handle):
     X = torch.eye(40, dtype=torch.float64, device=DEV)
     X[3, 5] = float("nan")

@@ -346,6 +346,30 @@ def test_lanczos_one_by_one(a):
     assert O.lanczos_two_norm(np.array([[a]]), 20, 1e-10, q0=np.ones(1), keep_exact=False) == (0.0, 1.0)
 
 
+def test_lanczos_keeps_krylov_completing_iteration():
+    # 3 x 3 input (rng 148) and the start vector of seed 9: the reference's
+    # convergence rule (src/lanczos.cu:145) drops the third iteration, which
+    # completes the Krylov space, and its bound then undercuts ||X||_2; the
+    # library keeps that iteration (keep_exact), so theta is ||X||_2^2.
+    from _streams import start_vector
+    rng = np.random.default_rng(148)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    lam = rng.uniform(-5.0, 5.0, 3)
+    X = (Q * lam) @ Q.T
+    X = 0.5 * (X + X.T)
+    nrm = np.linalg.norm(X, 2)
+    q0 = start_vector(9, 3)
+    lo0, up0 = O.lanczos_two_norm(X, 20, 1e-10, q0, keep_exact=False)
+    lo1, up1, st = O.lanczos_two_norm(X, 20, 1e-10, q0, return_state=True)
+    assert up0 < nrm
+    assert st["nb"] == 3
+    assert lo1 == pytest.approx(nrm, rel=1e-12) and up1 >= nrm
+    # a larger matrix is unaffected (the rule only applies at k = n - 1)
+    Y = np.diag(np.linspace(-2.0, 3.0, 40))
+    assert O.lanczos_two_norm(Y, 20, 1e-10, start_vector(9, 40)) == \
+        O.lanczos_two_norm(Y, 20, 1e-10, start_vector(9, 40), keep_exact=False)
+
+
 def test_lanczos_invariant_start_vector():
     # q0 = e_0 spans an eigenspace of diag(d): beta_0 = 0 at once
     d = np.array([-4.0, 1.0, 2.0, 3.0])
