@@ -296,6 +296,21 @@ def test_three_by_three_bound_covers_norm(handle, method):
     assert err <= (1e-3 if method == "mixed_mx" else 1e-4), err
 
 
+@pytest.mark.parametrize("method,bar", [("mixed_mx", 1.5e-3), ("composite_FP32", 1e-4)])
+def test_tiny_sizes(handle, method, bar):
+    # n = 1 .. 21: the bound's Krylov space completes (n <= max_iter), the
+    # tiles are mostly padding; error against ||X||_F (the positive part of a
+    # tiny spectrum can vanish), U[-5, 5] spectra with Haar bases
+    handle.set_seed(9)
+    print(f"tiny sizes {method}", flush=True)
+    for n in range(1, 22):
+        X, Q, lam = O.spectral_input(n, 100 + n, "uniform_cluster")
+        Y = P.project(handle, torch.tensor(X, dtype=torch.float32, device=DEV), method=method)
+        Pm = (Q * np.maximum(lam, 0.0)) @ Q.T
+        e = np.linalg.norm(Y.double().cpu().numpy() - Pm) / np.linalg.norm(X)
+        assert e <= bar, (n, e)
+
+
 def test_nonfinite_input_reported(handle):
     X = torch.eye(40, dtype=torch.float64, device=DEV)
     X[3, 5] = float("nan")
