@@ -379,6 +379,25 @@ def test_batched_accuracy_and_batch_invariance(handle, method, bar):
     assert torch.equal(Ym[0], Ys[23]) and torch.equal(Ym[-1], Ys[0])
 
 
+@pytest.mark.parametrize("method,bar", [("mixed", 1.5e-3), ("composite_FP32", 1e-4)])
+def test_batched_tiny_blocks(handle, method, bar):
+    # the small cones of an SDP (n = 1 .. 40, U[-5, 5] spectra): every block
+    # within the method's error of ||X||_F, and bitwise its result alone
+    sizes = list(range(1, 41))
+    data = [O.spectral_input(n, 600 + n, "uniform_cluster") for n in sizes]
+    Xs = [torch.tensor(X, dtype=torch.float32, device=DEV) for X, _, _ in data]
+    handle.set_seed(11)
+    print(f"batched tiny blocks {method}", flush=True)
+    Ys = P.project_batched(handle, Xs, method=method)
+    for (X, Q, lam), Y in zip(data, Ys):
+        Pm = (Q * np.maximum(lam, 0.0)) @ Q.T
+        e = np.linalg.norm(Y.double().cpu().numpy() - Pm) / np.linalg.norm(X)
+        assert e <= bar, (X.shape[0], e)
+    for k in (0, 2, 19, 39):
+        handle.set_seed(11)
+        assert torch.equal(P.project(handle, Xs[k], method=method), Ys[k]), sizes[k]
+
+
 def test_batched_rejects_bad_blocks(handle):
     with pytest.raises(P.PsdpError):
         P.project_batched(handle, [])
