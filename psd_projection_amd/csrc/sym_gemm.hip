@@ -1651,6 +1651,8 @@ __global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const
   const int S = d.ksplit;
   auto src = gbl((const float*)d.part) + (int64_t)t.mat * S * 65536 + part * SPLIT_REDUCE_ROWS * 256;
   constexpr int G = 4, KB = 4;
+  static_assert(SPLIT_REDUCE_ROWS % 32 == 0 && (SPLIT_REDUCE_ROWS * 64) % (256 * G) == 0,
+                "whole F16MX blocks per workgroup; every quad index of the loop in range");
   typedef __attribute__((address_space(1))) const f32x4 gf4;
   for (int q0 = threadIdx.x; q0 < SPLIT_REDUCE_ROWS * 64; q0 += 256 * G) {
     f32x4 acc[G];
