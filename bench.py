@@ -122,8 +122,8 @@ def measured_traffic(n, fmt):
 
 
 def precision_study(args, P, W, torch, h, X, Y, Pm, out):
-    """Precision study at n=16384 (uniform_cluster spectrum): the FP32-table chain with FP32 MFMA, BF16x3 and BF16x9 split
-    products, and e4m3 operands (per-tensor scale 2^8) on the first half of the
+    """Precision study at n=16384 (uniform_cluster spectrum): the FP32-table
+    chain with FP32 MFMA, BF16x3 and BF16x9 split products, and e4m3 operands (per-tensor scale 2^8) on the first half of the
     steps with bfloat16 on the rest; next to the shipped schedule.  e4m3
     operand values are computed on the binary16 MFMA (products of e4m3 values
     are exact there), so its time is that of binary16, not of FP8 MFMA."""
@@ -188,8 +188,9 @@ def chain_mfma_ms(method, sizes):
 
 
 def bench_batched(args, P, W, torch):
-    """The block-diagonal batch: 256 blocks, n log-uniform in [32, 1024] (seed 4), lambda ~ U[-1,1],
-    one grouped launch per filter-chain product for all blocks."""
+    """The block-diagonal batch: 256 blocks, n log-uniform in [32, 1024]
+    (seed 4), lambda ~ U[-1,1], one grouped launch per filter-chain product
+    for all blocks."""
     import numpy as np
     rng = np.random.default_rng(4)
     sizes = [int(round(float(np.exp(rng.uniform(np.log(32), np.log(1024)))))) for _ in range(args.batched)]
