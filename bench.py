@@ -123,8 +123,9 @@ def measured_traffic(n, fmt):
 
 def precision_study(args, P, W, torch, h, X, Y, Pm, out):
     """Precision study at n=16384 (uniform_cluster spectrum): the FP32-table
-    chain with FP32 MFMA, BF16x3 and BF16x9 split products, and e4m3 operands (per-tensor scale 2^8) on the first half of the
-    steps with bfloat16 on the rest; next to the shipped schedule.  e4m3
+    chain with FP32 MFMA, BF16x3 and BF16x9 split products, and e4m3 operands
+    (per-tensor scale 2^8) on the first half of the steps with bfloat16 on the
+    rest; next to the shipped schedule.  e4m3
     operand values are computed on the binary16 MFMA (products of e4m3 values
     are exact there), so its time is that of binary16, not of FP8 MFMA."""
     variants = [
