@@ -457,6 +457,9 @@ def main():
                     help="configs 1 (n=512) and 2 (n=8192, FP32 vs FP16 chain) (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of capacity")
     ap.add_argument("--serve-lanes", type=int, default=1, help="lanes (streams) for the n > 1024 requests")
+    ap.add_argument("--serve-alt-lanes", type=int, default=2,
+                    help="lanes of a second, shorter served run (capacity against latency; 0: skip)")
+    ap.add_argument("--serve-alt-requests", type=int, default=3000, help="requests of the second served run")
     ap.add_argument("--serve-ticks-ahead", type=int, default=4,
                     help="session ticks the served loop keeps enqueued ahead of the GPU")
     ap.add_argument("--deflate", type=int, default=1024, help="size of the deflated-projection leg (0: skip)")
@@ -619,6 +622,12 @@ def main():
         out["batched"] = bench_batched(args, P, W, torch)
     if args.serve_requests > 0:
         out["served"] = bench_served(args, P, W, torch)
+        if args.serve_alt_lanes > 0 and args.serve_alt_lanes != args.serve_lanes and args.serve_alt_requests > 0:
+            # the other side of the capacity / latency trade: more lanes side by side
+            import copy
+            alt = copy.copy(args)
+            alt.serve_lanes, alt.serve_requests = args.serve_alt_lanes, args.serve_alt_requests
+            out["served_alt_lanes"] = bench_served(alt, P, W, torch)
     if args.deflate > 0:
         out["deflate"] = bench_deflate(args, P, W, torch)
 
