@@ -61,7 +61,7 @@ def _drive(sess, waves, max_steps=20000, out=None):
     return out, order, steps
 
 
-@pytest.mark.parametrize("method", ["mixed", "composite_FP32"])
+@pytest.mark.parametrize("method", ["mixed", "mixed_mx", "composite_FP32"])
 def test_session_joins_and_matches_alone(method):
     sizes = [64, 300, 1024, 33, 700, 128, 1, 512, 90, 2100, 256]
     xs, ps = _inputs(sizes)
@@ -79,7 +79,7 @@ def test_session_joins_and_matches_alone(method):
         torch.cuda.synchronize()
         assert torch.equal(alone, y), f"n={n} differs from its alone result"
         err = W.rel_fro_error(y, pm)
-        assert err <= (1e-3 if method == "mixed" else 2e-5), (n, err)
+        assert err <= (1e-3 if method in ("mixed", "mixed_mx") else 2e-5), (n, err)
     sess.close()
 
 
