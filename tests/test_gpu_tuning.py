@@ -21,13 +21,14 @@ def blocks(sizes, seed0):
     return xs, ps
 
 
-@pytest.mark.parametrize("n", [1500, 2304])
-def test_autotuned_projection_equals_default(n):
+@pytest.mark.parametrize("method,n", [("mixed", 1500), ("mixed", 2304),
+                                      ("mixed_mx", 2304)])
+def test_autotuned_projection_equals_default(method, n):
     X64, Pm, _ = W.spectral_matrix(n, 31, "normal")
     X = X64.float().contiguous()
     h = P.Handle(0, seed=3)
-    cfg = P.default_config("mixed")
-    print(f"autotune n={n}", flush=True)
+    cfg = P.default_config(method)
+    print(f"autotune {method} n={n}", flush=True)
     Y0 = P.project(h, X, config=cfg)
     torch.cuda.synchronize()
     choice = h.autotune(n, config=cfg)
@@ -40,20 +41,22 @@ def test_autotuned_projection_equals_default(n):
     h.close()
 
 
-def test_batched_autotune_and_tilings_agree():
+@pytest.mark.parametrize("method", ["mixed", "mixed_mx"])
+def test_batched_autotune_and_tilings_agree(method):
     sizes = [40, 100, 300, 520, 700, 64, 900, 1000, 256, 33]
     xs, ps = blocks(sizes, 900)
     h = P.Handle(0, seed=4)
-    ref = P.project_batched(h, xs)
+    print(f"batched autotune {method}", flush=True)
+    ref = P.project_batched(h, xs, method=method)
     torch.cuda.synchronize()
     outs = [torch.empty_like(x) for x in xs]
-    h.autotune_batched(xs, outs)
+    h.autotune_batched(xs, outs, method=method)
     torch.cuda.synchronize()
     for a, b in zip(ref, outs):
         assert torch.equal(a, b)
     for big_n, conc in ((256, True), (512, False), (100000, False)):
         h.set_batched_tiling(big_n, conc)
-        o = P.project_batched(h, xs)
+        o = P.project_batched(h, xs, method=method)
         torch.cuda.synchronize()
         for a, b in zip(ref, o):
             assert torch.equal(a, b), (big_n, conc)
