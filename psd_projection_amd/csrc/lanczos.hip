@@ -317,12 +317,27 @@ __device__ void init_body(LanczosState* st, const LanczosVecs& v, int n, int m, 
 }
 
 // One Lanczos iteration after w = A (A q) (src/lanczos.cu:130-168); n is the
-// logical size of the matrix.
+// logical size of the matrix.  With parts = true, w is first formed here from
+// the symmetric GEMV's partial sums (w[i] = sum_t P[t][i] in ascending t, the
+// sum symv_reduce_body forms, so w is bitwise the same) instead of by a
+// reduction launch: the batched bound saves one launch per iteration.
+template <bool parts>
 __device__ void step_body(LanczosState* st, const LanczosVecs& v, int k, int n, int m, int max_iter, double tol) {
   __shared__ double red[16];
   if (st->done || k >= max_iter) return;
   double s = 0.0;
-  for (int i = threadIdx.x; i < m; i += blockDim.x) s += v.q[i] * v.w[i];
+  if constexpr (parts) {
+    const int nt = (m + SYMV_TILE - 1) / SYMV_TILE;
+    const auto Pg = gbl(v.part);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double wi = 0.0;
+      for (int t = 0; t < nt; ++t) wi += Pg[(int64_t)t * m + i];
+      v.w[i] = wi;
+      s += v.q[i] * wi;
+    }
+  } else {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) s += v.q[i] * v.w[i];
+  }
   const double alpha = block_sum(s, red);
   const double mbeta_old = -st->beta_old;
   s = 0.0;
@@ -525,14 +540,27 @@ __device__ __forceinline__ void ritz_vector_body(const LanczosState* st, const L
 // tight: r = A (A y) - theta y (v.w1 holds A A y), the residual of the Ritz
 // pair of A^T A itself, so that up = sqrt(theta + ||r||) exceeds ||A||_2 by
 // the Ritz pair's own error instead of by |lambda - lambda^2|.
+// With parts = true, the last GEMV's result (ay, or w1 for the tight bound)
+// is first formed here from its partial sums, as in step_body.
+template <bool parts>
 __device__ void bound_body(LanczosState* st, const LanczosVecs& v, int m, double margin, int tight) {
   __shared__ double red[16];
   if (st->skip_ritz) return;
   const double th = st->theta;
-  const double* ry = tight ? v.w1 : v.ay;
+  double* ry = tight ? v.w1 : v.ay;
   double s = 0.0;
+  const int nt = (m + SYMV_TILE - 1) / SYMV_TILE;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double r = fma(-th, v.y[i], ry[i]);
+    double ri;
+    if constexpr (parts) {
+      const auto Pg = gbl(v.part);
+      ri = 0.0;
+      for (int t = 0; t < nt; ++t) ri += Pg[(int64_t)t * m + i];
+      ry[i] = ri;
+    } else {
+      ri = ry[i];
+    }
+    const double r = fma(-th, v.y[i], ri);
     s += r * r;
   }
   const double rn = sqrt(block_sum(s, red));
@@ -557,7 +585,7 @@ __global__ __launch_bounds__(1024) void lanczos_init_kernel(LanczosState* st, La
 }
 __global__ __launch_bounds__(1024) void lanczos_step_kernel(LanczosState* st, LanczosVecs v, int k, int n, int m,
                                                             int max_iter, double tol) {
-  step_body(st, v, k, n, m, max_iter, tol);
+  step_body<false>(st, v, k, n, m, max_iter, tol);
 }
 __global__ __launch_bounds__(64) void lanczos_ritz_kernel(LanczosState* st) { ritz_body(st); }
 __global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st, LanczosVecs v, int m) {
@@ -565,7 +593,7 @@ __global__ __launch_bounds__(256) void ritz_vector_kernel(const LanczosState* st
 }
 __global__ __launch_bounds__(1024) void lanczos_bound_kernel(LanczosState* st, LanczosVecs v, int m,
                                                              double margin, int tight) {
-  bound_body(st, v, m, margin, tight);
+  bound_body<false>(st, v, m, margin, tight);
 }
 
 // ------------------------------------------------------------------ batched
@@ -575,7 +603,7 @@ __global__ __launch_bounds__(1024) void lanczos_init_batched(const LanczosJob* j
 }
 __global__ __launch_bounds__(1024) void lanczos_step_batched(const LanczosJob* jobs, int k, double tol) {
   const LanczosJob& J = jobs[blockIdx.x];
-  step_body(J.st, J.v, k, J.n, J.m, J.max_iter, tol);
+  step_body<true>(J.st, J.v, k, J.n, J.m, J.max_iter, tol);
 }
 __global__ __launch_bounds__(64) void lanczos_ritz_batched(const LanczosJob* jobs) { ritz_body(jobs[blockIdx.x].st); }
 __global__ __launch_bounds__(256) void ritz_vector_batched(const LanczosJob* jobs) {
@@ -584,7 +612,7 @@ __global__ __launch_bounds__(256) void ritz_vector_batched(const LanczosJob* job
 }
 __global__ __launch_bounds__(1024) void lanczos_bound_batched(const LanczosJob* jobs, double margin, int tight) {
   const LanczosJob& J = jobs[blockIdx.x];
-  bound_body(J.st, J.v, J.m, margin, tight);
+  bound_body<true>(J.st, J.v, J.m, margin, tight);
 }
 
 // Fixed scale (no bound): used by the non-auto-scaled entry points.
@@ -662,17 +690,19 @@ hipError_t launch_lanczos_batched(const LanczosJob* jobs, int count, int max_m, 
   const dim3 tgrid(ntiles), rgrid(nchunks);
   hipLaunchKernelGGL(lanczos_init_batched, dim3(count), dim3(1024), 0, s, jobs, seed);
   for (int k = 0; k < max_iter; ++k) {
-    for (int which = 0; which < 2; ++which) {
-      hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, which);
-      hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, which);
-    }
+    hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, 0);
+    hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, 0);
+    hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, 1);
+    // w = sum of the partial sums: formed by the step itself
     hipLaunchKernelGGL(lanczos_step_batched, dim3(count), dim3(1024), 0, s, jobs, k, tol);
   }
   hipLaunchKernelGGL(lanczos_ritz_batched, dim3(count), dim3(64), 0, s, jobs);
   hipLaunchKernelGGL(ritz_vector_batched, dim3((max_m + 255) / 256, count), dim3(256), 0, s, jobs);
-  for (int which = 2; which < (tight ? 4 : 3); ++which) {
+  const int last = tight ? 3 : 2;
+  for (int which = 2; which <= last; ++which) {
     hipLaunchKernelGGL(symv_tile_batched, tgrid, dim3(256), 0, s, jobs, tiles, which);
-    hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, which);
+    // the last result's partial sums are added by the bound kernel itself
+    if (which < last) hipLaunchKernelGGL(symv_reduce_batched, rgrid, dim3(256), 0, s, jobs, chunks, which);
   }
   hipLaunchKernelGGL(lanczos_bound_batched, dim3(count), dim3(1024), 0, s, jobs, margin, tight);
   return hipGetLastError();
