@@ -231,13 +231,15 @@ def bench_batched(args, P, W, torch):
         res["mfma_frac"] = round(flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["f16"], 4)
         # HBM bytes a batch cannot avoid with one launch per product: per padded
         # element, each product's epilogue writes its result (pieces; float32
-        # copies of A^3 and of the new A for the polynomial) and the polynomial
-        # reads A and A^3 in float32, and each product reads its two operands
-        # once; 41 Lanczos GEMVs read the upper triangle of the float32 matrix.
+        # copies of the new A and, except in binary16x3 steps, whose polynomial
+        # reads A^3 back from its two pieces, of A^3) and the polynomial reads A
+        # and A^3, and each product reads its two operands once; 41 Lanczos
+        # GEMVs read the upper triangle of the float32 matrix.
         npad2 = sum(float(((n + 63) // 64 * 64) ** 2) for n in sizes)
         lanczos_bytes = 41.0 * sum(symv_bytes((n + 63) // 64 * 64) for n in sizes)
-        per_step = {"f16x3": 4 + (4 + 4) + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
-        per_step["f16mx"] = per_step["f16x3"]  # binary16 hi + e4m3 hi and lo bytes (+ 1/16 B of scales)
+        per_step = {"f16x3": 4 + 4 + (8 + 4 + 4) + 4 * 2 + 4 * 2 + 4, "f16": 2 + (4 + 2) + (8 + 4 + 2) + 2 + 4 + 4}
+        # binary16 hi + e4m3 hi and lo bytes (+ 1/16 B of scales), and the float32 A^3
+        per_step["f16mx"] = per_step["f16x3"] + 4
         steps = SCHEDULE_FMTS[method]
         bytes_batch = npad2 * (sum(per_step[f] for f in steps) + 2 + 2 + 4) + lanczos_bytes
         t_hbm = bytes_batch / 8.0e12 * 1e3
@@ -245,7 +247,7 @@ def bench_batched(args, P, W, torch):
                            "model_ms_mfma": round(t_mfma, 3),
                            "frac_of_bound": round(max(t_hbm, t_mfma) / ms, 4),
                            "model_bytes": bytes_batch,
-                           "note": "blocks n <= 1024: each product's epilogue traffic (~30 B per element) "
+                           "note": "blocks n <= 1024: each product's epilogue traffic (~20-30 B per element) "
                                    "outweighs its 2n flops per element, so the batch is HBM-bound"}
     log(f"batched: {res}")
     return res

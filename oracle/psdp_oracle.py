@@ -185,6 +185,13 @@ def mx_split(x, axis):
     return [hi, mx_e4m3(w, axis), mx_e4m3(w - hi, axis)]
 
 
+def f16x3_value(x):
+    """float32 value of a binary16x3 operand entry from its pieces,
+    (hi + lo) 2^-8 (within 2^-22 |x| + 2^-33 of x)."""
+    hi, lo = split_operand(x, "f16x3")
+    return ((hi + lo) * np.float32(1.0 / F16X3_SCALE)).astype(np.float32)
+
+
 def split_operand(x, fmt):
     """Return the list of float32 pieces a GEMM operand is stored as."""
     x = np.asarray(x, dtype=np.float32)
@@ -296,7 +303,10 @@ def composite_chain(X, coeffs, step_formats, final_format=None, sym_mode="refere
         a, b, c = dt(a), dt(b), dt(c)
         A2 = prod(A, A, fmt)
         A3 = prod(A2, A, fmt)
-        A = a * A + b * A3
+        # a binary16x3 step reads A^3 for the b term back from its two
+        # operand pieces, (hi + lo) 2^-8, instead of a float32 copy
+        A3b = f16x3_value(A3) if fmt == "f16x3" else A3
+        A = a * A + b * A3b
         A = A + c * prod(A3, A2, fmt)
         if sym_mode == "reference":
             A = _sym(A)
@@ -853,6 +863,8 @@ def spectral_input(n, seed, kind):
 
 # How each oracle function is pinned (checked in tests/test_oracle.py).
 PINNING = {
+    "f16x3_value": "within 2^-22 |x| + 2^-33 of x on [-1, 1] (the split's own bound); exact on values hi + lo "
+                   "already on the binary16x3 grid (tests/test_oracle.py)",
     "step_coefficients": "float32 re-derivation + transfer-function KATs",
     "transfer_function": "max|g-ReLU| on [-1,1] (4.48e-6 FP32 table, 4.65e-5 FP16 table), "
                          "oddness of P, g(x)=x for x in [1e-2,1], g(x)=0 for x in [-1,-1e-2]",

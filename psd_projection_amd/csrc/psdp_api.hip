@@ -557,7 +557,8 @@ static bool desc_ok(const ProdDesc& d, int fmt, const char** why) {
       break;
     case EPI_POLY:
     case EPI_POLY_HALF_ID:
-      if (!d.c32 || !d.p3) return *why = "missing A or A^3", false;
+      // binary16x3 steps read A^3 back from the pieces of X (p3 null)
+      if (!d.c32 || (!d.p3 && fmt != FMT_F16X3)) return *why = "missing A or A^3", false;
       break;
     case EPI_OUTPUT:
       if (!d.out || d.out_ld < d.n) return *why = "missing output", false;
@@ -795,6 +796,15 @@ static int tuned_sub(psdp_handle_t h, const Plan& p, int fmt, int avg) {
 // composite_* functions, the deflated matrix's scale for the *_deflate ones).
 // st: device Lanczos state (default: the handle's); ft: tile lists in caller
 // memory (default: the handle's cached lists).
+// Whether a step in operand format fmt keeps a float32 copy of A^3 for its
+// polynomial epilogue.  A binary16x3 step does not: the epilogue reads A^3
+// back from its two pieces (within 2^-22 |v| + 2^-33 of the float32 value v,
+// the precision the products already take it at), which saves the A^3 product a
+// float32 store of every entry.  Other formats keep the copy (one binary16 or
+// bfloat16 piece is too coarse; the float32, FP8 and MX formats have no
+// pieces to read it from).
+static bool a3_f32(int fmt) { return fmt != FMT_F16X3; }
+
 static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bufs& b, const void* X, int x_f64,
                                         void* Y, int y_f64, const psdp_config_t* cfg, double fixed_scale = 1.0,
                                         LanczosState* st = nullptr, const FixedTiles* ft = nullptr) {
@@ -845,7 +855,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     set_operand(d2.x, f, b.p2, b.p2p);
     set_operand(d2.y, f, b.a32, b.ap);
     d2.mode = EPI_STORE;
-    d2.c32 = b.p3;
+    d2.c32 = a3_f32(f) ? b.p3 : nullptr;
     d2.out_fmt = f;
     for (int q = 0; q < 3; ++q) d2.outp[q] = b.p3p[q];
     STCHK(product(f, sch.avg[i], d2));
@@ -858,7 +868,7 @@ static psdp_status_t enqueue_projection(psdp_handle_t h, const Plan& p, const Bu
     d3.b = sch.b[i];
     d3.c = sch.c[i];
     d3.c32 = b.a32;
-    d3.p3 = b.p3;
+    d3.p3 = a3_f32(f) ? b.p3 : nullptr;
     d3.out_fmt = fn;
     for (int q = 0; q < 3; ++q) d3.outp[q] = b.ap[q];
     STCHK(product(f, sch.avg[i], d3));
@@ -1618,7 +1628,7 @@ static void block_descs(const Schedule& sch, const BlockBufs& q, char* base, con
     set_operand(d2.x, f, B.p2, B.p2p);
     set_operand(d2.y, f, B.a32, B.ap);
     d2.mode = EPI_STORE;
-    d2.c32 = B.p3;
+    d2.c32 = a3_f32(f) ? B.p3 : nullptr;
     d2.out_fmt = f;
     for (int k = 0; k < 3; ++k) d2.outp[k] = B.p3p[k];
     ProdDesc& d3 = pd[(size_t)(3 * i + 2) * stride];
@@ -1630,7 +1640,7 @@ static void block_descs(const Schedule& sch, const BlockBufs& q, char* base, con
     d3.b = sch.b[i];
     d3.c = sch.c[i];
     d3.c32 = B.a32;
-    d3.p3 = B.p3;
+    d3.p3 = a3_f32(f) ? B.p3 : nullptr;
     d3.out_fmt = fn;
     for (int k = 0; k < 3; ++k) d3.outp[k] = B.ap[k];
   }

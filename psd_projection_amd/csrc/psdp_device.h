@@ -33,6 +33,12 @@ __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
 __device__ __forceinline__ float f16_bits_to_f32(uint16_t b) {
   return (float)__builtin_bit_cast(_Float16, b);
 }
+// Value of a binary16x3 operand entry from its pieces, (hi + lo) 2^-8: within
+// 2^-22 |v| + 2^-33 of the float32 value v it was split from (the absolute
+// term where lo falls into the binary16 subnormals).
+__device__ __forceinline__ float f16x3_value(uint16_t hi, uint16_t lo) {
+  return (f16_bits_to_f32(hi) + f16_bits_to_f32(lo)) * (1.0f / F16X3_SCALE);
+}
 // Branch-free: NaN keeps its sign and top payload bits with the quiet bit set;
 // infinities and finite values round to nearest even.
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {

@@ -66,7 +66,7 @@ struct ProdDesc {
   int out_fmt;       // operand format written by EPI_STORE / EPI_POLY*
   float a, b, c;     // polynomial coefficients (EPI_POLY*)
   float* c32;        // float32 result (EPI_STORE) or A in/out (EPI_POLY*)
-  const float* p3;   // float32 A^3 read by EPI_POLY*
+  const float* p3;   // float32 A^3 read by EPI_POLY* (binary16x3 products read A^3 from the pieces of x instead)
   uint16_t* outp[3]; // pieces written by EPI_STORE / EPI_POLY*
   void* out;         // EPI_OUTPUT destination (column-major n x n, ld = out_ld)
   int64_t out_ld;

@@ -132,15 +132,53 @@ __device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c, int fmt) {
 template <int BT>
 constexpr int NR_MAX = BT < 128 ? BT : 128;
 
+// A^3 quad of the polynomial epilogue at idx.  Products of binary16x3
+// operands (A3PCS) read it from the pieces xh / xl of their X (the A^3 of the
+// polynomial product), raw here and decoded by a3_value4 once loaded, so that
+// the loads of several quads stay in flight together; the other formats read
+// the float32 copy p3.  A compile-time choice: the other kernels' epilogues
+// stay as they are.
+template <bool A3PCS>
+__device__ __forceinline__ f32x4 a3_load4(__attribute__((address_space(1))) const float* p3,
+                                          __attribute__((address_space(1))) const uint16_t* xh,
+                                          __attribute__((address_space(1))) const uint16_t* xl, int64_t idx) {
+  if constexpr (A3PCS) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 h = *(const __attribute__((address_space(1))) u32x2*)(xh + idx);
+    const u32x2 l = *(const __attribute__((address_space(1))) u32x2*)(xl + idx);
+    // elements go through scalars: __builtin_bit_cast of a vector subscript
+    // (v[i]) reads element 0 whatever i is
+    const uint32_t h0 = h.x, h1 = h.y, l0 = l.x, l1 = l.y;
+    return f32x4{__builtin_bit_cast(float, h0), __builtin_bit_cast(float, h1), __builtin_bit_cast(float, l0),
+                 __builtin_bit_cast(float, l1)};
+  } else {
+    return *(const __attribute__((address_space(1))) f32x4*)(p3 + idx);
+  }
+}
+template <bool A3PCS>
+__device__ __forceinline__ f32x4 a3_value4(f32x4 raw) {
+  if constexpr (A3PCS) {
+    const float r0 = raw.x, r1 = raw.y, r2 = raw.z, r3 = raw.w;
+    const uint32_t h0 = __builtin_bit_cast(uint32_t, r0), h1 = __builtin_bit_cast(uint32_t, r1);
+    const uint32_t l0 = __builtin_bit_cast(uint32_t, r2), l1 = __builtin_bit_cast(uint32_t, r3);
+    return f32x4{f16x3_value((uint16_t)h0, (uint16_t)l0), f16x3_value((uint16_t)(h0 >> 16), (uint16_t)(l0 >> 16)),
+                 f16x3_value((uint16_t)h1, (uint16_t)l1), f16x3_value((uint16_t)(h1 >> 16), (uint16_t)(l1 >> 16))};
+  } else {
+    return raw;
+  }
+}
+
 // Primary and mirror passes of a polynomial / store epilogue with the output
 // operand format OF fixed at compile time, so that the per-element split is
 // straight-line code and the loops stay compact (the instruction cache holds
 // them; a runtime format switch unrolled per element did not fit).
-template <int BT, int NT, int OF, int GMAX = 8>
+template <int BT, int NT, int OF, int GMAX = 8, bool A3PCS = false>
 __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, float cscale, int row0, int col0,
                                                     bool diag, int64_t ld,
                                                     __attribute__((address_space(1))) float* c32,
                                                     __attribute__((address_space(1))) const float* p3,
+                                                    __attribute__((address_space(1))) const uint16_t* xh,
+                                                    __attribute__((address_space(1))) const uint16_t* xl,
                                                     uint16_t* const outp[3], bool poly, bool half_id, float ca,
                                                     float cb, float cc) {
   constexpr int CS = BT + 1;
@@ -183,7 +221,7 @@ __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, fl
         if (q < NR * Q4) {
           const int64_t idx = (int64_t)(row0 + q / Q4) * ld + col0 + (q % Q4) * 4;
           a4g[g] = *(const __attribute__((address_space(1))) f32x4*)(c32 + idx);
-          p4g[g] = *(const __attribute__((address_space(1))) f32x4*)(p3 + idx);
+          p4g[g] = a3_load4<A3PCS>(p3, xh, xl, idx);
         }
       }
     }
@@ -199,7 +237,7 @@ __device__ __forceinline__ void epilogue_store_rows(float* C, int r0, int NR, fl
       for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
       if (poly) {
         const f32x4 a4 = a4g[g];
-        const f32x4 p4 = p4g[g];
+        const f32x4 p4 = a3_value4<A3PCS>(p4g[g]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           float x = ca * a4[e];
@@ -269,11 +307,13 @@ __device__ __forceinline__ void mx_store1(uint16_t* const pieces[3], int64_t r, 
   *gbl(b8 + 32) = (char)(l8 >> (8 * e));
 }
 
-template <int BT, int NT, int GMAX = 8>
+template <int BT, int NT, int GMAX = 8, bool A3PCS = false>
 __device__ __forceinline__ void epilogue_store_rows_mx(float* C, int r0, int NR, float cscale, int row0, int col0,
                                                        bool diag, int64_t ld,
                                                        __attribute__((address_space(1))) float* c32,
                                                        __attribute__((address_space(1))) const float* p3,
+                                                       __attribute__((address_space(1))) const uint16_t* xh,
+                                                       __attribute__((address_space(1))) const uint16_t* xl,
                                                        uint16_t* const outp[3], bool poly, bool half_id, float ca,
                                                        float cb, float cc) {
   constexpr int CS = BT + 1;
@@ -294,7 +334,7 @@ __device__ __forceinline__ void epilogue_store_rows_mx(float* C, int r0, int NR,
         if (q < NR * Q4) {
           const int64_t idx = (int64_t)(row0 + q / Q4) * ld + col0 + (q % Q4) * 4;
           a4g[g] = *(const __attribute__((address_space(1))) f32x4*)(c32 + idx);
-          p4g[g] = *(const __attribute__((address_space(1))) f32x4*)(p3 + idx);
+          p4g[g] = a3_load4<A3PCS>(p3, xh, xl, idx);
         }
       }
     }
@@ -308,10 +348,11 @@ __device__ __forceinline__ void epilogue_store_rows_mx(float* C, int r0, int NR,
 #pragma unroll
       for (int e = 0; e < 4; ++e) w[e] = C[r * CS + c0 + e] * cscale;
       if (poly) {
+        const f32x4 p4 = a3_value4<A3PCS>(p4g[g]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           float x = ca * a4g[g][e];
-          x = fmaf(cb, p4g[g][e], x);
+          x = fmaf(cb, p4[e], x);
           x = fmaf(cc, w[e], x);
           if (half_id) x = (x + (i == j0 + e ? 1.0f : 0.0f)) * 0.5f;
           w[e] = x;
@@ -404,7 +445,7 @@ __device__ __forceinline__ void partial_rows(float* ptile, const float* C, int r
   }
 }
 
-template <int BT, int NT = 256, int GMAX = 8, bool MXOUT = true>
+template <int BT, int NT = 256, int GMAX = 8, bool MXOUT = true, bool A3PCS = false>
 __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj, float* C, int r0, int NR,
                                               float cscale = 1.0f) {
   constexpr int CS = BT + 1;
@@ -419,7 +460,9 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   // compiler would otherwise reload it after each store it cannot prove
   // disjoint from it.
   auto c32 = gbl(d.c32);
-  auto p3 = gbl(d.p3);
+  auto p3 = gbl(d.p3);  // A3PCS: A^3 from the pieces of X instead (binary16x3 products)
+  auto xh = gbl((const uint16_t*)d.x[0]);
+  auto xl = gbl((const uint16_t*)d.x[1]);
   const int mode = d.mode, out_fmt = d.out_fmt, n = d.n, out_f64 = d.out_f64;
   const float ca = d.a, cb = d.b, cc = d.c;
   uint16_t* const outp[3] = {d.outp[0], d.outp[1], d.outp[2]};
@@ -470,12 +513,12 @@ __device__ __forceinline__ void epilogue_rows(const ProdDesc& d, int bi, int bj,
   const bool poly = mode == EPI_POLY || mode == EPI_POLY_HALF_ID;
   const bool half_id = mode == EPI_POLY_HALF_ID;
 #define PSDP_EPI(OF)                                                                                      \
-  epilogue_store_rows<BT, NT, OF, GMAX>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id, ca, \
-                                  cb, cc)
+  epilogue_store_rows<BT, NT, OF, GMAX, A3PCS>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, xh, xl, outp,  \
+                                              poly, half_id, ca, cb, cc)
   if constexpr (MXOUT) {
     if (out_fmt == FMT_F16MX) {
-      epilogue_store_rows_mx<BT, NT, GMAX>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, outp, poly, half_id,
-                                           ca, cb, cc);
+      epilogue_store_rows_mx<BT, NT, GMAX, A3PCS>(C, r0, NR, cscale, row0, col0, diag, ld, c32, p3, xh, xl, outp,
+                                                  poly, half_id, ca, cb, cc);
       return;
     }
   }
@@ -697,7 +740,7 @@ __device__ __forceinline__ void sym_tile(const ProdDesc& d, int bi, int bj,
           }
     }
     __syncthreads();
-    epilogue_rows<TL::BT, 256, 8, false>(d, bi, bj, Ct, ph * TL::HR, TL::HR);
+    epilogue_rows<TL::BT, 256, 8, false, FMT == FMT_F16X3>(d, bi, bj, Ct, ph * TL::HR, TL::HR);
     __syncthreads();
   }
 }
@@ -1030,7 +1073,7 @@ __device__ __forceinline__ void sym_tile_dma(const ProdDesc& d, int bi, int bj, 
     }
     __syncthreads();
     if constexpr (PART) partial_rows<DT::BT, NT>(ptile, Ct, ph * DT::HR, DT::HR);
-    else epilogue_rows<DT::BT, NT>(d, bi, bj, Ct, ph * DT::HR, DT::HR, PSCALE);
+    else epilogue_rows<DT::BT, NT, 8, true, FMT == FMT_F16X3>(d, bi, bj, Ct, ph * DT::HR, DT::HR, PSCALE);
     __syncthreads();
   }
 }
@@ -1197,8 +1240,8 @@ __device__ __forceinline__ void sym_tile_dma8(const ProdDesc& d, const TileRef& 
     partial_rows<256, NT>(ptile, Ct, (PH) * 64, 64);                                              \
     partial_rows<256, NT>(ptile, Ct + 64 * CS, 128 + (PH) * 64, 64);                              \
   } else {                                                                                        \
-    epilogue_rows<256, NT, 2>(d, bi, bj, Ct, (PH) * 64, 64, PSCALE);                              \
-    epilogue_rows<256, NT, 2>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, PSCALE);              \
+    epilogue_rows<256, NT, 2, true, FMT == FMT_F16X3>(d, bi, bj, Ct, (PH) * 64, 64, PSCALE);      \
+    epilogue_rows<256, NT, 2, true, FMT == FMT_F16X3>(d, bi, bj, Ct + 64 * CS, 128 + (PH) * 64, 64, PSCALE); \
   }                                                                                               \
   __syncthreads();
   PSDP_STAGE8(0)
@@ -1642,6 +1685,7 @@ __global__ __launch_bounds__(512) void sym_prod_dma8_mx_kernel(ProdDesc d, const
 // reads do not wait one by one.  32 rows per workgroup keep every F16MX block
 // (32 values of a row, also of the mirrored rows) inside one workgroup.
 constexpr int SPLIT_REDUCE_ROWS = 32, SPLIT_REDUCE_WGS = 256 / SPLIT_REDUCE_ROWS;
+template <bool A3PCS>
 __global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const TileRef* __restrict__ tiles,
                                                                float cscale) {
   __shared__ __attribute__((aligned(16))) float Ct[SPLIT_REDUCE_ROWS * 257];
@@ -1696,7 +1740,7 @@ __global__ __launch_bounds__(256) void sym_split_reduce_kernel(ProdDesc d, const
     }
   }
   __syncthreads();
-  epilogue_rows<256, 256>(d, t.bi, t.bj, Ct, part * SPLIT_REDUCE_ROWS, SPLIT_REDUCE_ROWS, cscale);
+  epilogue_rows<256, 256, 8, true, A3PCS>(d, t.bi, t.bj, Ct, part * SPLIT_REDUCE_ROWS, SPLIT_REDUCE_ROWS, cscale);
 }
 
 template <int SUB>
@@ -1903,7 +1947,10 @@ hipError_t launch_sym_product(int fmt, int sub, int avg, const ProdDesc& d, cons
 
 hipError_t launch_split_reduce(int fmt, const ProdDesc& d, const TileRef* tiles, int ntiles, hipStream_t s) {
   const float cscale = (fmt == FMT_F16X3 || fmt == FMT_E4M3 || fmt == FMT_F16MX) ? F16X3_PROD_SCALE : 1.0f;
-  hipLaunchKernelGGL(sym_split_reduce_kernel, dim3(SPLIT_REDUCE_WGS * ntiles), dim3(256), 0, s, d, tiles, cscale);
+  if (fmt == FMT_F16X3)
+    hipLaunchKernelGGL(sym_split_reduce_kernel<true>, dim3(SPLIT_REDUCE_WGS * ntiles), dim3(256), 0, s, d, tiles, cscale);
+  else
+    hipLaunchKernelGGL(sym_split_reduce_kernel<false>, dim3(SPLIT_REDUCE_WGS * ntiles), dim3(256), 0, s, d, tiles, cscale);
   return hipGetLastError();
 }
 

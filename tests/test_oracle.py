@@ -512,3 +512,19 @@ def test_eig_fp64_staged_matches_eigh(kind):
     Y5, w5, _ = O.eig_FP64_psd_staged(np.diag(k["diagonal"]))
     assert np.abs(Y5 - np.diag(k["expected_diagonal"])).max() < 1e-14
     assert np.allclose(w5, np.sort(k["diagonal"]))
+
+
+def test_f16x3_value_bound_and_exact_grid():
+    # A^3 of a binary16x3 step is read back from its pieces: (hi + lo) 2^-8
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, 200000).astype(np.float32)
+    v = O.f16x3_value(x)
+    # relative 2^-22 from the 11 + 11 bits, absolute 2^-33 where lo falls into
+    # the binary16 subnormals (quantum 2^-24, scaled back by 2^-8)
+    assert np.all(np.abs(v.astype(np.float64) - x) <= 2.0 ** -22 * np.abs(x) + 2.0 ** -33)
+    # values already on the grid come back exactly, and a dropped lo piece
+    # (a plausible mistake) is caught by the bound
+    hi, lo = O.split_operand(x, "f16x3")
+    g = ((hi + lo) / np.float32(256.0)).astype(np.float32)
+    assert np.array_equal(O.f16x3_value(g), g)
+    assert np.any(np.abs(hi / np.float32(256.0) - x) > 2.0 ** -22 * np.abs(x))
