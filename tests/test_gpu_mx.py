@@ -130,3 +130,49 @@ def test_mixed_mx_session_equals_batched(handle):
         torch.cuda.synchronize()
         assert torch.equal(alone, y)
     sess.close()
+
+
+# Alternating binary16x3 and F16MX steps: the binary16x3 polynomial epilogue
+# writing F16MX pieces (A^3 read back from its binary16x3 pieces, A from
+# float32) and the F16MX epilogue writing binary16x3 pieces, on the 64 x 64
+# kernels and (n = 2304) on the 8-wave 256 x 256 kernels, bitwise the same.
+ALT_STEPS = ["f16x3", "f16mx", "f16x3", "f16mx", "f16", "f16", "f16"]
+ALT = dict(method="custom", table="fp16", step_format=ALT_STEPS, final_format="f16")
+
+
+@pytest.mark.parametrize("n", [600, 2304])
+def test_alternating_f16x3_f16mx_schedule(handle, n):
+    from _streams import start_vector
+    seed = 41
+    print(f"alternating f16x3/f16mx schedule n={n}", flush=True)
+    X, Q, lam = O.spectral_input(n, 6, "uniform_cluster")
+    X32 = X.astype(np.float32)
+    Xd = torch.from_numpy(X32).to(DEV)
+    fmts = ("f16x3", "f16mx", "f16")
+
+    def run(tile):
+        for f in fmts:
+            handle.set_tile(n, f, tile)
+        try:
+            handle.set_seed(seed)
+            return P.project(handle, Xd, **ALT)
+        finally:
+            for f in fmts:
+                handle.set_tile(n, f, 0)
+
+    Y = run(64)
+    if n >= 2304:
+        # 9 x 9 tiles of 256, one partial round: every tiling sums each entry
+        # in the same order
+        assert torch.equal(run(256), Y)
+    Yn = Y.cpu().numpy().astype(np.float64)
+    assert np.array_equal(Yn, Yn.T)
+    Pt = O.closed_form_projection(Q, lam)
+    err = O.rel_fro_error(Yn, Pt)
+    _, up = O.lanczos_two_norm(X32.astype(np.float64), 20, 1e-10, q0=start_vector(seed, n))
+    c16 = O.fp16_chain_coefficients()
+    Yo = O.composite_chain(X32.astype(np.float64) / up, c16, ALT_STEPS, "f16", sym_mode="mirror") * up
+    e_or = O.rel_fro_error(Yo, Pt)
+    print(f"  gpu {err:.3e} oracle {e_or:.3e}", flush=True)
+    assert 0.7 * e_or - 2e-6 <= err <= 1.3 * e_or + 2e-6
+    assert err <= 1e-3
