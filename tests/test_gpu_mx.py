@@ -176,3 +176,26 @@ def test_alternating_f16x3_f16mx_schedule(handle, n):
     print(f"  gpu {err:.3e} oracle {e_or:.3e}", flush=True)
     assert 0.7 * e_or - 2e-6 <= err <= 1.3 * e_or + 2e-6
     assert err <= 1e-3
+
+
+def test_all_f16x3_schedule_half_identity_epilogue(handle):
+    # every step binary16x3 and a binary16x3 final product: the last
+    # polynomial epilogue ((I + A)/2) also reads A^3 back from its pieces
+    from _streams import start_vector
+    n, seed = 500, 43
+    steps = ["f16x3"] * 7
+    X, Q, lam = O.spectral_input(n, 7, "uniform_cluster")
+    X32 = X.astype(np.float32)
+    handle.set_seed(seed)
+    Y = P.project(handle, torch.from_numpy(X32).to(DEV), method="custom", table="fp16", step_format=steps,
+                  final_format="f16x3")
+    Yn = Y.cpu().numpy().astype(np.float64)
+    Pt = O.closed_form_projection(Q, lam)
+    err = O.rel_fro_error(Yn, Pt)
+    _, up = O.lanczos_two_norm(X32.astype(np.float64), 20, 1e-10, q0=start_vector(seed, n))
+    Yo = O.composite_chain(X32.astype(np.float64) / up, O.fp16_chain_coefficients(), steps, "f16x3",
+                           sym_mode="mirror") * up
+    e_or = O.rel_fro_error(Yo, Pt)
+    print(f"all-f16x3 schedule n={n}: gpu {err:.3e} oracle {e_or:.3e}", flush=True)
+    assert 0.7 * e_or - 2e-6 <= err <= 1.3 * e_or + 2e-6
+    assert err <= 1e-3
